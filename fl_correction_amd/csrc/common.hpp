@@ -1,0 +1,73 @@
+// Shared definitions for the MI355X (gfx950) kernels of the XRF self-absorption
+// hot path.  Everything here is device-side HIP written for CDNA4: 64-lane
+// wavefronts, 160 KB LDS per CU, 8 XCDs with private L2.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <cstdint>
+#include <cstdio>
+#include <string>
+
+#include "../../include/fl_correction_amd.h"
+
+namespace fl {
+
+constexpr int kWave = 64;
+constexpr int kNumXcd = 8;
+
+// Thread-local message behind fl_last_error().
+void set_error(const std::string &msg);
+
+// Remap the hardware workgroup id so that the workgroups dispatched to one XCD
+// (the dispatcher deals them round-robin: id % 8) cover one contiguous range of
+// logical ids.  Logical ids are laid out so that neighbours share input data
+// (the same slice of C, the same image tile), which then stays in that XCD's L2.
+__device__ __forceinline__ uint32_t xcd_remap(uint32_t id, uint32_t n)
+{
+    const uint32_t x = id % kNumXcd, l = id / kNumXcd;
+    const uint32_t q = n / kNumXcd, r = n % kNumXcd;
+    return x < r ? x * (q + 1) + l : r * (q + 1) + (x - r) * q + l;
+}
+
+__device__ __forceinline__ float clampf(float v, float lo, float hi)
+{
+    return __builtin_amdgcn_fmed3f(v, lo, hi);
+}
+
+// Non-finite angles are read as 0: every sample coordinate then stays finite and
+// every index stays inside the arrays.
+__device__ __forceinline__ float finite_or_zero(float v) { return isfinite(v) ? v : 0.0f; }
+
+__device__ __forceinline__ int uniform_int(int v) { return __builtin_amdgcn_readfirstlane(v); }
+
+__device__ __forceinline__ float uniform_float(float v)
+{
+    return __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(v)));
+}
+
+}  // namespace fl
+
+#define FL_HIP_CHECK(expr)                                                            \
+    do {                                                                              \
+        hipError_t _e = (expr);                                                       \
+        if (_e != hipSuccess) {                                                       \
+            fl::set_error(std::string(#expr) + ": " + hipGetErrorString(_e));         \
+            return FL_ERR_HIP;                                                        \
+        }                                                                             \
+    } while (0)
+
+#define FL_LAUNCH_CHECK()                                                             \
+    do {                                                                              \
+        hipError_t _e = hipGetLastError();                                            \
+        if (_e != hipSuccess) {                                                       \
+            fl::set_error(std::string("kernel launch: ") + hipGetErrorString(_e));    \
+            return FL_ERR_HIP;                                                        \
+        }                                                                             \
+    } while (0)
+
+#define FL_REQUIRE(cond, msg)                                                         \
+    do {                                                                              \
+        if (!(cond)) {                                                                \
+            fl::set_error(msg);                                                       \
+            return FL_ERR_INVALID_ARG;                                                \
+        }                                                                             \
+    } while (0)
