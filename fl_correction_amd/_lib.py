@@ -30,14 +30,22 @@ _INT = ctypes.c_int
 SIGNATURES = {
     "fl_last_error": (ctypes.c_char_p, []),
     "fl_version": (_INT, []),
-    "fl_forward_emission_batch": (_INT, [_P, _P, _P, _P, _P, _I64, _I64, _I64, _I64, _I64, _P, _SZ, _P]),
+    "fl_forward_emission_batch": (_INT, [_P, _P, _P, _P, _P, _I64, _I64, _I64, _I64, _I64, _P]),
     "fl_backward_emission_batch": (_INT, [_P, _P, _P, _P, _P, _I64, _I64, _I64, _I64, _I64, _P]),
-    "fl_forward_emission": (_INT, [_P, _P, _P, _P, _P, _I64, _I64, _I64, _I64, _P, _SZ, _P]),
+    "fl_forward_emission": (_INT, [_P, _P, _P, _P, _P, _I64, _I64, _I64, _I64, _P]),
     "fl_backward_emission": (_INT, [_P, _P, _P, _P, _P, _I64, _I64, _I64, _I64, _P]),
-    "fl_projector_workspace_bytes": (_SZ, [_I64, _I64, _I64, _I64]),
     "fl_huber_grad_batch": (_INT, [_P, _P, _I64, _I64, _I64, _I64, _F, _P]),
     "fl_mlem_batch": (_INT, [_P, _P, _P, _P, _P, _P, _I64, _I64, _I64, _I64, _I64, _I64, _D, _D, _P, _SZ, _P]),
     "fl_mlem_workspace_bytes": (_SZ, [_I64, _I64, _I64, _I64, _I64]),
+    "fl_mlem_sensitivity": (_INT, [_P, _P, _P, _P, _I64, _I64, _I64, _I64, _I64, _P]),
+    "fl_mlem_ratio": (_INT, [_P, _P, _P, _P, _P, _P, _I64, _I64, _I64, _I64, _I64, _P]),
+    "fl_mlem_update": (_INT, [_P, _P, _P, _P, _P, _P, _P, _I64, _I64, _I64, _I64, _I64, _D, _D, _P]),
+    "fl_cal_atten": (_INT, [_P, _P, _P, _I64, _I64, _I64, _I64, _I64, _I64, _I64, _P]),
+    "fl_rotate": (_INT, [_P, _P, _I64, _I64, _I64, _F, _F, _P]),
+    "fl_incident_atten": (_INT, [_P, _P, _P, _I64, _I64, _I64, _I64, _I64, _I64, _P]),
+    "fl_mu_xrf_rot": (_INT, [_P, _P, _P, _I64, _I64, _I64, _I64, _I64, _INT, _P]),
+    "fl_cone_atten": (_INT, [_P, _P, _I64, _I64, _I64, _I64, _P, _P, _P, _P, _I64, _P]),
+    "fl_xrf_combine": (_INT, [_P, _P, _P, _I64, _I64, _I64, _I64, _I64, _I64, _INT, _P]),
 }
 
 

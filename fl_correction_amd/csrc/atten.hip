@@ -1,0 +1,407 @@
+// Attenuation maps for CDNA4: the incident-beam product, the fluorescence-path
+// line integral over the detector cone, the in-plane rotation of the volume and
+// the frame changes between them.
+//
+// Frames.  The original frame is (B, H, W) = (slice, row, col); the incident
+// beam runs along rows toward row 0 (FL/torch_functions.py:95-102) and the
+// detector sits at +col.  The mask frame of atten_kernel (FL/cuda_lib/
+// atten_cuda_kernel.cu:10-57) is (Z, X, Y) = (slice, col, H-1-row): the
+// reference reaches it with torch_rot90_4D(..., 'clock') (torch_functions.py:143)
+// and the cone extends along +X.  Here mu is written straight into that frame
+// by the assembly kernel and read back, transposed, by the combine kernel.
+//
+// Kernels
+//   mask_corr_kernel   out = M (*) mu, any dense mask, zero taps skipped
+//   rotate_kernel      affine_grid + grid_sample rotation (zeros, align_corners=False)
+//   incident_kernel    prod_{m>r} (1 - frac*coef) down each column
+//   mu_rot_kernel      mu_e = sum_j frac_j*coef[j,e], written in the mask frame
+//   cone_kernel        one sampled cone direction: exact rational-step recursion
+//   combine_kernel     exp(-T) (optionally times the incident product), back to (B,H,W)
+#include "common.hpp"
+
+#include <algorithm>
+#include <cmath>
+#include <numeric>
+#include <vector>
+
+namespace fl {
+
+// ---------------------------------------------------------------------------
+// Dense mask correlation (atten_kernel, .cu:10-57):
+//   out[k,z,x,y] = sum_{dz,dx,dy} mask[dz,dx,dy] * mu[k, z+dz-Mz/2, x+dx, y+dy-My/2]
+// One thread computes kXB outputs along x for one (k, z, y); lanes run along y,
+// so every mu load is coalesced and each loaded value feeds kXB multiply-adds.
+// Mask values are wave-uniform (scalar loads); zero taps are skipped.
+// ---------------------------------------------------------------------------
+constexpr int kXB = 8;
+
+__global__ __launch_bounds__(256) void mask_corr_kernel(const float *__restrict__ mu, const float *__restrict__ mask,
+                                                        float *__restrict__ out, int K, int Z, int X, int Y, int Mz,
+                                                        int Mx, int My, int nxb)
+{
+    const int y = blockIdx.x * 64 + (threadIdx.x & 63);
+    const int xb = blockIdx.y * 4 + (threadIdx.x >> 6);
+    const int kz = blockIdx.z;
+    const int k = kz / Z, z = kz % Z;
+    if (xb >= nxb) return;
+    const int x0 = xb * kXB;
+    const int hz = Mz / 2, hy = My / 2;
+    const int64_t XY = (int64_t)X * Y;
+    const float *muk = mu + (int64_t)k * Z * XY;
+    float acc[kXB];
+#pragma unroll
+    for (int j = 0; j < kXB; ++j) acc[j] = 0.0f;
+
+    for (int dz = 0; dz < Mz; ++dz) {
+        const int zz = z + dz - hz;
+        if (zz < 0 || zz >= Z) continue;
+        const float *plane = muk + (int64_t)zz * XY;
+        for (int dy = 0; dy < My; ++dy) {
+            const int yy = y + dy - hy;
+            const bool yok = yy >= 0 && yy < Y;
+            const float *mcol = mask + (int64_t)dz * Mx * My + dy;  // mask[dz, :, dy], stride My
+            const int mlast = min(kXB + Mx - 1, X - x0);
+            for (int m = 0; m < mlast; ++m) {
+                const float v = yok ? plane[(int64_t)(x0 + m) * Y + yy] : 0.0f;
+#pragma unroll
+                for (int j = 0; j < kXB; ++j) {
+                    const int dx = m - j;
+                    if (dx >= 0 && dx < Mx) {
+                        const float mv = mcol[(int64_t)dx * My];
+                        if (mv != 0.0f) acc[j] = fmaf(mv, v, acc[j]);
+                    }
+                }
+            }
+        }
+    }
+    if (y >= Y) return;
+    float *o = out + (int64_t)k * Z * XY + (int64_t)z * XY + y;
+#pragma unroll
+    for (int j = 0; j < kXB; ++j)
+        if (x0 + j < X) o[(int64_t)(x0 + j) * Y] = acc[j];
+}
+
+// ---------------------------------------------------------------------------
+// Rotation: F.affine_grid([[c,-s,0],[s,c,0]]) + F.grid_sample(bilinear, zeros,
+// align_corners=False), as torch_rotate_image_nd (torch_image_util.py:337-438).
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void rotate_kernel(const float *__restrict__ x, float *__restrict__ out, int64_t N,
+                                                     int H, int W, float c, float s)
+{
+    const int j = blockIdx.x * 64 + (threadIdx.x & 63);
+    const int i = blockIdx.y * 4 + (threadIdx.x >> 6);
+    if (j >= W || i >= H) return;
+    for (int64_t n = blockIdx.z; n < N; n += gridDim.z) {
+    const float xn = (float)(2 * j + 1) / (float)W - 1.0f;
+    const float yn = (float)(2 * i + 1) / (float)H - 1.0f;
+    const float gx = c * xn - s * yn;
+    const float gy = s * xn + c * yn;
+    const float ix = ((gx + 1.0f) * W - 1.0f) * 0.5f;
+    const float iy = ((gy + 1.0f) * H - 1.0f) * 0.5f;
+    const float fx0 = floorf(ix), fy0 = floorf(iy);
+    const int x0 = (int)fx0, y0 = (int)fy0;
+    const float e = ix - fx0, w_ = 1.0f - e;
+    const float d = iy - fy0, n_ = 1.0f - d;
+    const float *img = x + n * H * W;
+    float v = 0.0f;
+    const bool x0ok = x0 >= 0 && x0 < W, x1ok = x0 + 1 >= 0 && x0 + 1 < W;
+    const bool y0ok = y0 >= 0 && y0 < H, y1ok = y0 + 1 >= 0 && y0 + 1 < H;
+    if (y0ok && x0ok) v += img[(int64_t)y0 * W + x0] * (w_ * n_);
+    if (y0ok && x1ok) v += img[(int64_t)y0 * W + x0 + 1] * (e * n_);
+    if (y1ok && x0ok) v += img[(int64_t)(y0 + 1) * W + x0] * (w_ * d);
+    if (y1ok && x1ok) v += img[(int64_t)(y0 + 1) * W + x0 + 1] * (e * d);
+    out[n * H * W + (int64_t)i * W + j] = v;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Incident beam (torch_xray_atten_3D, torch_functions.py:92-102):
+//   A[e,b,H-1,c] = 1,  A[e,b,r,c] = prod_{m=r+1}^{H-1} (1 - frac[e,b,m,c]*coef[e])
+// coef == nullptr: frac already holds mu_x.  The running product starts at row
+// H-1 and multiplies toward row 0 (the order of cumprod over the flipped rows).
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void incident_kernel(const float *__restrict__ frac, const float *__restrict__ coef,
+                                                       float *__restrict__ out, int64_t frac_estride,
+                                                       int64_t out_estride, int K, int B, int H, int W)
+{
+    const int c = blockIdx.x * 256 + threadIdx.x;
+    const int b = blockIdx.y, e = blockIdx.z;
+    if (c >= W) return;
+    const int64_t HW = (int64_t)H * W;
+    const float *f = frac + e * frac_estride + b * HW + c;
+    float *o = out + e * out_estride + b * HW + c;
+    const float k = coef ? coef[e] : 1.0f;
+    o[(int64_t)(H - 1) * W] = 1.0f;
+    float p = 1.0f;
+    for (int r = H - 2; r >= 0; --r) {
+        const float mu = coef ? f[(int64_t)(r + 1) * W] * k : f[(int64_t)(r + 1) * W];
+        const float t = 1.0f - mu;
+        p = (r == H - 2) ? t : p * t;
+        o[(int64_t)r * W] = p;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Fluorescence-line attenuation coefficients in the mask frame
+// (torch_extract_3D_mu_xrf, torch_functions.py:71-81, then rot90 'clock'):
+//   mu_rot[e, b, c, H-1-r] = sum_j frac[j,b,r,c] * coef[j*K + e]
+// Tile 32 rows x 64 cols through LDS so both the read and the write coalesce.
+// ---------------------------------------------------------------------------
+constexpr int kTR = 32, kTC = 64, kMaxLines = 4;
+
+// Offset of original-frame voxel (r, c) inside one (X, Y) plane of the mask frame.
+// side 0, detector at +col ('r', rot90 'clock'):      X = c,       Y = H-1-r
+// side 1, detector at -col ('l', rot90 'c-clock'):    X = W-1-c,   Y = r
+__device__ __forceinline__ int64_t mask_frame_offset(int r, int c, int H, int W, int side)
+{
+    return side == 0 ? (int64_t)c * H + (H - 1 - r) : (int64_t)(W - 1 - c) * H + r;
+}
+
+__global__ __launch_bounds__(256) void mu_rot_kernel(const float *__restrict__ frac, const float *__restrict__ coef,
+                                                     float *__restrict__ mu_rot, int Kc, int K, int B, int H, int W,
+                                                     int side)
+{
+    __shared__ float tile[kMaxLines][kTR][kTC + 1];
+    const int c0 = blockIdx.x * kTC, r0 = blockIdx.y * kTR;
+    const int b = blockIdx.z % B, echunk = blockIdx.z / B;
+    const int e0 = echunk * kMaxLines, ne = min(kMaxLines, K - e0);
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int64_t HW = (int64_t)H * W, BHW = (int64_t)B * HW;
+    for (int rr = wave; rr < kTR; rr += 4) {
+        const int r = r0 + rr, c = c0 + lane;
+        float acc[kMaxLines];
+#pragma unroll
+        for (int e = 0; e < kMaxLines; ++e) acc[e] = 0.0f;
+        if (r < H && c < W) {
+            for (int j = 0; j < Kc; ++j) {
+                const float fj = frac[j * BHW + b * HW + (int64_t)r * W + c];
+#pragma unroll
+                for (int e = 0; e < kMaxLines; ++e)
+                    if (e < ne) acc[e] = acc[e] + fj * coef[j * K + e0 + e];
+            }
+        }
+#pragma unroll
+        for (int e = 0; e < kMaxLines; ++e)
+            if (e < ne) tile[e][rr][lane] = acc[e];
+    }
+    __syncthreads();
+    // write: for column c, the mask-frame row index is y = H-1-r (contiguous in mu_rot)
+    const int half = lane >> 5, rr = lane & 31;
+    for (int e = 0; e < ne; ++e)
+        for (int cc = wave * 2 + half; cc < kTC; cc += 8) {
+            const int c = c0 + cc, r = r0 + rr;
+            if (c < W && r < H)
+                mu_rot[(int64_t)(e0 + e) * BHW + b * HW + mask_frame_offset(r, c, H, W, side)] = tile[e][rr][cc];
+        }
+}
+
+// ---------------------------------------------------------------------------
+// One direction of the sampled detector cone, mask frame (Z, X, Y), cone along +X.
+//   J(v) = sum_{t>=1} bilin0(mu[., x+t, .], z + t*pz/q, y + t*py/q)
+// With D = (pz, q, py) integer, J(v) = sum_{t=1}^{q} bilin0(...) + J(v + D): every
+// voxel lies on one lattice chain v0 + m*D, and each thread walks one chain from
+// its far end, so the recursion is exact (no re-interpolation of J).  The q-step
+// stencil (<= 4q taps with exact rational weights) comes from the host.
+//   T[v] (+)= weight * J(v)
+// ---------------------------------------------------------------------------
+constexpr int kMaxTaps = 128;
+
+struct ConeTaps {
+    int n;
+    int pz, py, q;
+    float weight;
+    int accumulate;
+    int zlo, nz, ylo, ny;  // chain start ranges (mask frame)
+    int8_t t[kMaxTaps];
+    int8_t dz[kMaxTaps];
+    int8_t dy[kMaxTaps];
+    float w[kMaxTaps];
+};
+
+__global__ __launch_bounds__(256) void cone_kernel(const float *__restrict__ mu, float *__restrict__ T, int K, int Z,
+                                                   int X, int Y, ConeTaps p)
+{
+    const int yi = blockIdx.x * 64 + (threadIdx.x & 63);
+    const int zi = blockIdx.y * 4 + (threadIdx.x >> 6);
+    const int x0 = blockIdx.z % p.q;
+    const int k = blockIdx.z / p.q;
+    if (yi >= p.ny || zi >= p.nz) return;
+    const int z0 = p.zlo + zi, y0 = p.ylo + yi;
+    const int64_t XY = (int64_t)X * Y;
+    const float *muk = mu + (int64_t)k * Z * XY;
+    float *Tk = T + (int64_t)k * Z * XY;
+    const int M = (X - x0 + p.q - 1) / p.q;  // chain points with x < X
+    float J = 0.0f;
+    for (int m = M - 1; m >= 0; --m) {
+        const int x = x0 + m * p.q;
+        const int z = z0 + m * p.pz;
+        const int y = y0 + m * p.py;
+        // whole stencil outside the transverse range (plus the bilinear reach)?
+        const int zmin = z - 2 * (abs(p.pz) + 1), zmax = z + 2 * (abs(p.pz) + 1);
+        const int ymin = y - 2 * (abs(p.py) + 1), ymax = y + 2 * (abs(p.py) + 1);
+        if (zmax < 0 || zmin >= Z || ymax < 0 || ymin >= Y) {
+            // the ray from here stays outside: only J beyond matters, which is
+            // already in J (zero if the chain has not entered yet)
+            continue;
+        }
+        float S = 0.0f;
+        for (int i = 0; i < p.n; ++i) {
+            const int xx = x + p.t[i];
+            const int zz = z + p.dz[i];
+            const int yy = y + p.dy[i];
+            if (xx < X && zz >= 0 && zz < Z && yy >= 0 && yy < Y)
+                S = fmaf(p.w[i], muk[(int64_t)zz * XY + (int64_t)xx * Y + yy], S);
+        }
+        J += S;
+        if (z >= 0 && z < Z && y >= 0 && y < Y) {
+            float *o = Tk + (int64_t)z * XY + (int64_t)x * Y + y;
+            *o = p.accumulate ? fmaf(p.weight, J, *o) : p.weight * J;
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// out[e,b,r,c] = exp(-T_rot[e,b,c,H-1-r]) * (incident ? incident[e,b,r,c] : 1)
+// (torch_functions.py:170-174, then core.py:99 for the product), via an LDS tile.
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void combine_kernel(const float *__restrict__ T_rot,
+                                                      const float *__restrict__ incident, float *__restrict__ out,
+                                                      int64_t inc_estride, int64_t out_estride, int B, int H, int W,
+                                                      int side)
+{
+    __shared__ float tile[kTR][kTC + 1];
+    const int c0 = blockIdx.x * kTC, r0 = blockIdx.y * kTR;
+    const int b = blockIdx.z % B, e = blockIdx.z / B;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int64_t HW = (int64_t)H * W;
+    const float *Te = T_rot + (int64_t)e * B * HW + b * HW;
+    const int half = lane >> 5, rr = lane & 31;
+    for (int cc = wave * 2 + half; cc < kTC; cc += 8) {
+        const int c = c0 + cc, r = r0 + rr;
+        if (c < W && r < H) tile[rr][cc] = Te[mask_frame_offset(r, c, H, W, side)];
+    }
+    __syncthreads();
+    for (int r_ = wave; r_ < kTR; r_ += 4) {
+        const int r = r0 + r_, c = c0 + lane;
+        if (r >= H || c >= W) continue;
+        float v = expf(-tile[r_][lane]);
+        if (incident) v *= incident[e * inc_estride + b * HW + (int64_t)r * W + c];
+        out[e * out_estride + b * HW + (int64_t)r * W + c] = v;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Host side
+// ---------------------------------------------------------------------------
+static int64_t floordiv(int64_t a, int64_t b)
+{
+    int64_t q = a / b, r = a % b;
+    return (r != 0 && ((r < 0) != (b < 0))) ? q - 1 : q;
+}
+
+int launch_mask_corr(const float *mu, const float *mask, float *out, int64_t K, int64_t Z, int64_t X, int64_t Y,
+                     int64_t Mz, int64_t Mx, int64_t My, hipStream_t s)
+{
+    const int nxb = (int)((X + kXB - 1) / kXB);
+    dim3 grid((unsigned)((Y + 63) / 64), (unsigned)((nxb + 3) / 4), (unsigned)(K * Z));
+    FL_REQUIRE(K * Z <= 65535, "mask correlation: K*Z must be <= 65535");
+    hipLaunchKernelGGL(mask_corr_kernel, grid, dim3(256), 0, s, mu, mask, out, (int)K, (int)Z, (int)X, (int)Y,
+                       (int)Mz, (int)Mx, (int)My, nxb);
+    FL_LAUNCH_CHECK();
+    return FL_OK;
+}
+
+int launch_rotate(const float *x, float *out, int64_t N, int64_t H, int64_t W, float c, float s_, hipStream_t s)
+{
+    dim3 grid((unsigned)((W + 63) / 64), (unsigned)((H + 3) / 4), (unsigned)std::min<int64_t>(N, 65535));
+    hipLaunchKernelGGL(rotate_kernel, grid, dim3(256), 0, s, x, out, N, (int)H, (int)W, c, s_);
+    FL_LAUNCH_CHECK();
+    return FL_OK;
+}
+
+int launch_incident(const float *frac, const float *coef, float *out, int64_t frac_estride, int64_t out_estride,
+                    int64_t K, int64_t B, int64_t H, int64_t W, hipStream_t s)
+{
+    dim3 grid((unsigned)((W + 255) / 256), (unsigned)B, (unsigned)K);
+    hipLaunchKernelGGL(incident_kernel, grid, dim3(256), 0, s, frac, coef, out, frac_estride, out_estride, (int)K,
+                       (int)B, (int)H, (int)W);
+    FL_LAUNCH_CHECK();
+    return FL_OK;
+}
+
+int launch_mu_rot(const float *frac, const float *coef, float *mu_rot, int64_t Kc, int64_t K, int64_t B, int64_t H,
+                  int64_t W, int side, hipStream_t s)
+{
+    const int64_t echunks = (K + kMaxLines - 1) / kMaxLines;
+    dim3 grid((unsigned)((W + kTC - 1) / kTC), (unsigned)((H + kTR - 1) / kTR), (unsigned)(B * echunks));
+    hipLaunchKernelGGL(mu_rot_kernel, grid, dim3(256), 0, s, frac, coef, mu_rot, (int)Kc, (int)K, (int)B, (int)H,
+                       (int)W, side);
+    FL_LAUNCH_CHECK();
+    return FL_OK;
+}
+
+int launch_combine(const float *T_rot, const float *incident, float *out, int64_t inc_estride, int64_t out_estride,
+                   int64_t K, int64_t B, int64_t H, int64_t W, int side, hipStream_t s)
+{
+    dim3 grid((unsigned)((W + kTC - 1) / kTC), (unsigned)((H + kTR - 1) / kTR), (unsigned)(B * K));
+    hipLaunchKernelGGL(combine_kernel, grid, dim3(256), 0, s, T_rot, incident, out, inc_estride, out_estride, (int)B,
+                       (int)H, (int)W, side);
+    FL_LAUNCH_CHECK();
+    return FL_OK;
+}
+
+// Builds the q-step stencil of direction (pz, py, q) and launches one chain walk.
+int launch_cone_direction(const float *mu, float *T, int64_t K, int64_t Z, int64_t X, int64_t Y, int pz, int py,
+                          int q, float weight, int accumulate, hipStream_t s)
+{
+    FL_REQUIRE(q >= 1, "cone direction: q must be >= 1");
+    // reduce the rational slopes so the chain step is as short as possible
+    int g = std::gcd(std::gcd(std::abs(pz), std::abs(py)), q);
+    if (g > 1) {
+        pz /= g;
+        py /= g;
+        q /= g;
+    }
+    FL_REQUIRE(4 * q <= kMaxTaps, "cone direction: reduced denominator too large (max 32)");
+    FL_REQUIRE(std::abs(pz) <= 64 * q && std::abs(py) <= 64 * q, "cone direction: slope too steep");
+    ConeTaps p{};
+    p.pz = pz;
+    p.py = py;
+    p.q = q;
+    p.weight = weight;
+    p.accumulate = accumulate;
+    int n = 0;
+    for (int t = 1; t <= q; ++t) {
+        const int64_t nz = (int64_t)t * pz, ny = (int64_t)t * py;
+        const int64_t z0 = floordiv(nz, q), y0 = floordiv(ny, q);
+        const double fz = (double)(nz - z0 * q) / q, fy = (double)(ny - y0 * q) / q;
+        for (int a = 0; a < 2; ++a)
+            for (int b = 0; b < 2; ++b) {
+                const double wz = a ? fz : 1.0 - fz, wy = b ? fy : 1.0 - fy;
+                if (wz == 0.0 || wy == 0.0) continue;
+                FL_REQUIRE(n < kMaxTaps && std::abs(z0 + a) < 127 && std::abs(y0 + b) < 127,
+                           "cone direction: stencil too large");
+                p.t[n] = (int8_t)t;
+                p.dz[n] = (int8_t)(z0 + a);
+                p.dy[n] = (int8_t)(y0 + b);
+                p.w[n] = (float)(wz * wy);
+                ++n;
+            }
+    }
+    p.n = n;
+    const int64_t Mmax = (X + q - 1) / q;
+    // chain starts in plane x0 in [0,q): z0 = z - m*pz for z in [0,Z), m in [0,Mmax);
+    // one extra voxel each side for the bilinear reach
+    const int64_t zlo = -(Mmax - 1) * std::max(pz, 0) - 1, zhi = Z - 1 - (Mmax - 1) * std::min(pz, 0) + 1;
+    const int64_t ylo = -(Mmax - 1) * std::max(py, 0) - 1, yhi = Y - 1 - (Mmax - 1) * std::min(py, 0) + 1;
+    p.zlo = (int)zlo;
+    p.nz = (int)(zhi - zlo + 1);
+    p.ylo = (int)ylo;
+    p.ny = (int)(yhi - ylo + 1);
+    dim3 grid((unsigned)((p.ny + 63) / 64), (unsigned)((p.nz + 3) / 4), (unsigned)(K * q));
+    hipLaunchKernelGGL(cone_kernel, grid, dim3(256), 0, s, mu, T, (int)K, (int)Z, (int)X, (int)Y, p);
+    FL_LAUNCH_CHECK();
+    return FL_OK;
+}
+
+}  // namespace fl

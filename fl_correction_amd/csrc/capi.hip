@@ -11,17 +11,24 @@ namespace fl {
 static thread_local std::string g_last_error;
 void set_error(const std::string &msg) { g_last_error = msg; }
 
-int launch_forward(const float *atten, const float *C, const float *em, const float *theta,
-                   float *part, int64_t A, int64_t B, int64_t R, int64_t H, int64_t W,
-                   hipStream_t stream, int group);
-int launch_forward_reduce(const float *part, const float *I, float *out, int64_t A, int64_t B,
-                          int64_t H, int64_t W, int mode, hipStream_t stream);
+int launch_forward(const float *atten, const float *C, const float *em, const float *theta, const float *I,
+                   float *out, int64_t A, int64_t B, int64_t R, int64_t H, int64_t W, int mode, hipStream_t stream);
 int launch_backward(const float *atten, const float *ratio, const float *em, const float *theta,
                     float *out, int64_t A, int64_t B, int64_t R, int64_t H, int64_t W, int mode,
                     const float *C, const float *sens, float beta, float delta, hipStream_t stream);
 int launch_huber(const float *C, float *grad, int64_t n_slices, int64_t H, int64_t W, float delta,
                  hipStream_t stream);
-size_t forward_part_floats(int64_t A, int64_t B, int64_t H, int64_t W);
+int launch_mask_corr(const float *mu, const float *mask, float *out, int64_t K, int64_t Z, int64_t X, int64_t Y,
+                     int64_t Mz, int64_t Mx, int64_t My, hipStream_t s);
+int launch_rotate(const float *x, float *out, int64_t N, int64_t H, int64_t W, float c, float s_, hipStream_t s);
+int launch_incident(const float *frac, const float *coef, float *out, int64_t frac_estride, int64_t out_estride,
+                    int64_t K, int64_t B, int64_t H, int64_t W, hipStream_t s);
+int launch_mu_rot(const float *frac, const float *coef, float *mu_rot, int64_t Kc, int64_t K, int64_t B, int64_t H,
+                  int64_t W, int side, hipStream_t s);
+int launch_combine(const float *T_rot, const float *incident, float *out, int64_t inc_estride, int64_t out_estride,
+                   int64_t K, int64_t B, int64_t H, int64_t W, int side, hipStream_t s);
+int launch_cone_direction(const float *mu, float *T, int64_t K, int64_t Z, int64_t X, int64_t Y, int pz, int py,
+                          int q, float weight, int accumulate, hipStream_t s);
 
 static int check_dims(int64_t A, int64_t B, int64_t R, int64_t H, int64_t W)
 {
@@ -42,29 +49,15 @@ const char *fl_last_error(void) { return g_last_error.c_str(); }
 
 int fl_version(void) { return (1 << 16) | 0; }
 
-size_t fl_projector_workspace_bytes(int64_t A, int64_t B, int64_t H, int64_t W)
-{
-    return sizeof(float) * forward_part_floats(A, B, H, W);
-}
-
 int fl_forward_emission_batch(const float *atten, const float *C, const float *em_cs,
                               const float *theta, float *Pf, int64_t A, int64_t B, int64_t R,
-                              int64_t H, int64_t W, void *workspace, size_t workspace_bytes,
-                              void *stream)
+                              int64_t H, int64_t W, void *stream)
 {
     g_last_error.clear();
     if (int e = check_dims(A, B, R, H, W)) return e;
     if (A == 0 || B == 0) return FL_OK;
     FL_REQUIRE(atten && C && em_cs && theta && Pf, "null pointer argument");
-    const size_t need = fl_projector_workspace_bytes(A, B, H, W);
-    if (workspace_bytes < need || !workspace) {
-        set_error("workspace too small: need " + std::to_string(need) + " bytes");
-        return FL_ERR_WORKSPACE;
-    }
-    hipStream_t s = (hipStream_t)stream;
-    float *part = (float *)workspace;
-    if (int e = launch_forward(atten, C, em_cs, theta, part, A, B, R, H, W, s, 8)) return e;
-    return launch_forward_reduce(part, nullptr, Pf, A, B, H, W, 0, s);
+    return launch_forward(atten, C, em_cs, theta, nullptr, Pf, A, B, R, H, W, 0, (hipStream_t)stream);
 }
 
 int fl_backward_emission_batch(const float *atten, const float *ratio, const float *em_cs,
@@ -86,10 +79,9 @@ int fl_backward_emission_batch(const float *atten, const float *ratio, const flo
 
 int fl_forward_emission(const float *atten, const float *C, const float *em_cs,
                         const float *theta, float *Pf, int64_t A, int64_t R, int64_t H,
-                        int64_t W, void *workspace, size_t workspace_bytes, void *stream)
+                        int64_t W, void *stream)
 {
-    return fl_forward_emission_batch(atten, C, em_cs, theta, Pf, A, 1, R, H, W, workspace,
-                                     workspace_bytes, stream);
+    return fl_forward_emission_batch(atten, C, em_cs, theta, Pf, A, 1, R, H, W, stream);
 }
 
 int fl_backward_emission(const float *atten, const float *ratio, const float *em_cs,
@@ -108,13 +100,53 @@ int fl_huber_grad_batch(const float *C, float *grad, int64_t R, int64_t B, int64
     return launch_huber(C, grad, R * B, H, W, delta, (hipStream_t)stream);
 }
 
+int fl_mlem_sensitivity(const float *atten, const float *em_cs, const float *theta, float *sens,
+                        int64_t A, int64_t B, int64_t R, int64_t H, int64_t W, void *stream)
+{
+    g_last_error.clear();
+    if (int e = check_dims(A, B, R, H, W)) return e;
+    FL_REQUIRE(A >= 1, "sensitivity needs at least one angle");
+    if (B == 0) return FL_OK;
+    FL_REQUIRE(atten && em_cs && theta && sens, "null pointer argument");
+    // sens = max(B^T 1, 1e-6)   (atten_cuda.cpp:308-314)
+    return launch_backward(atten, nullptr, em_cs, theta, sens, A, B, R, H, W, 1, nullptr, nullptr, 0.f,
+                           0.f, (hipStream_t)stream);
+}
+
+int fl_mlem_ratio(const float *atten, const float *C, const float *em_cs, const float *theta,
+                  const float *I, float *ratio, int64_t A, int64_t B, int64_t R, int64_t H,
+                  int64_t W, void *stream)
+{
+    g_last_error.clear();
+    if (int e = check_dims(A, B, R, H, W)) return e;
+    if (A == 0 || B == 0) return FL_OK;
+    FL_REQUIRE(atten && C && em_cs && theta && I && ratio, "null pointer argument");
+    // Pf = forward(C); ratio = I / max(Pf, 1e-6)   (atten_cuda.cpp:323-328), one kernel
+    return launch_forward(atten, C, em_cs, theta, I, ratio, A, B, R, H, W, 2, (hipStream_t)stream);
+}
+
+int fl_mlem_update(const float *atten, const float *ratio, const float *em_cs, const float *theta,
+                   const float *C, const float *sens, float *C_next, int64_t A, int64_t B,
+                   int64_t R, int64_t H, int64_t W, double beta, double delta, void *stream)
+{
+    g_last_error.clear();
+    if (int e = check_dims(A, B, R, H, W)) return e;
+    FL_REQUIRE(A >= 1, "update needs at least one angle");
+    if (B == 0) return FL_OK;
+    FL_REQUIRE(atten && ratio && em_cs && theta && C && sens && C_next, "null pointer argument");
+    FL_REQUIRE(C_next != C, "C_next must not alias C (neighbours of C are read by other workgroups)");
+    // back = B^T ratio; C_next = max(0, C*back/(sens + beta*huber(C)))   (atten_cuda.cpp:330-341)
+    return launch_backward(atten, ratio, em_cs, theta, C_next, A, B, R, H, W, 2, C, sens, (float)beta,
+                           (float)delta, (hipStream_t)stream);
+}
+
 size_t fl_mlem_workspace_bytes(int64_t A, int64_t B, int64_t R, int64_t H, int64_t W)
 {
     const size_t nC = (size_t)(R * B * H * W);
     const size_t nP = (size_t)(A * B * W);
-    // sens, ping-pong estimate, forward partials, ratio; each region 256-byte aligned
+    // sens, ping-pong estimate, ratio; each region 256-byte aligned
     auto al = [](size_t f) { return (f * sizeof(float) + 255) / 256 * 256; };
-    return al(nC) + al(nC) + al(forward_part_floats(A, B, H, W)) + al(nP);
+    return al(nC) + al(nC) + al(nP);
 }
 
 int fl_mlem_batch(const float *C_init, const float *atten, const float *em_cs,
@@ -135,27 +167,20 @@ int fl_mlem_batch(const float *C_init, const float *atten, const float *em_cs,
     }
     hipStream_t s = (hipStream_t)stream;
     const size_t nC = (size_t)(R * B * H * W);
-    const size_t nP = (size_t)(A * B * W);
     auto al = [](size_t f) { return (f * sizeof(float) + 255) / 256 * 256; };
     char *ws = (char *)workspace;
     float *sens = (float *)ws;
     float *other = (float *)(ws + al(nC));
-    float *part = (float *)(ws + 2 * al(nC));
-    float *ratio = (float *)(ws + 2 * al(nC) + al(forward_part_floats(A, B, H, W)));
-    (void)nP;
+    float *ratio = (float *)(ws + 2 * al(nC));
 
-    // sens = max(B^T 1, 1e-6)   (atten_cuda.cpp:308-314)
-    if (int e = launch_backward(atten, nullptr, em_cs, theta, sens, A, B, R, H, W, 1, nullptr,
-                                nullptr, 0.f, 0.f, s))
-        return e;
-    // C = C_init.clone()  (:303); iterate between C_out and `other`
+    if (int e = fl_mlem_sensitivity(atten, em_cs, theta, sens, A, B, R, H, W, stream)) return e;
+    // C = C_init.clone()  (atten_cuda.cpp:303); iterate between C_out and `other`
     FL_HIP_CHECK(hipMemcpyAsync(C_out, C_init, nC * sizeof(float), hipMemcpyDeviceToDevice, s));
     float *cur = C_out, *nxt = other;
     for (int64_t it = 0; it < n_iter; ++it) {
-        if (int e = launch_forward(atten, cur, em_cs, theta, part, A, B, R, H, W, s, 8)) return e;
-        if (int e = launch_forward_reduce(part, I, ratio, A, B, H, W, 1, s)) return e;
-        if (int e = launch_backward(atten, ratio, em_cs, theta, nxt, A, B, R, H, W, 2, cur, sens,
-                                    (float)beta, (float)delta, s))
+        if (int e = fl_mlem_ratio(atten, cur, em_cs, theta, I, ratio, A, B, R, H, W, stream)) return e;
+        if (int e = fl_mlem_update(atten, ratio, em_cs, theta, cur, sens, nxt, A, B, R, H, W, beta,
+                                   delta, stream))
             return e;
         float *tmp = cur;
         cur = nxt;
@@ -164,6 +189,82 @@ int fl_mlem_batch(const float *C_init, const float *atten, const float *em_cs,
     if (cur != C_out)
         FL_HIP_CHECK(hipMemcpyAsync(C_out, cur, nC * sizeof(float), hipMemcpyDeviceToDevice, s));
     return FL_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Attenuation maps
+// ---------------------------------------------------------------------------
+
+int fl_cal_atten(const float *mu, const float *mask, float *out, int64_t K, int64_t Z, int64_t X, int64_t Y,
+                 int64_t Mz, int64_t Mx, int64_t My, void *stream)
+{
+    g_last_error.clear();
+    FL_REQUIRE(K >= 0 && Z >= 0 && X >= 0 && Y >= 0 && Mz >= 1 && Mx >= 1 && My >= 1, "shape: negative size");
+    FL_REQUIRE(X < (1 << 24) && Y < (1 << 24) && Z < (1 << 24), "shape: dimension too large");
+    if (K == 0 || Z == 0 || X == 0 || Y == 0) return FL_OK;
+    FL_REQUIRE(mu && mask && out, "null pointer argument");
+    FL_REQUIRE(mu != out, "out must not alias mu");
+    return launch_mask_corr(mu, mask, out, K, Z, X, Y, Mz, Mx, My, (hipStream_t)stream);
+}
+
+int fl_rotate(const float *x, float *out, int64_t N, int64_t H, int64_t W, float cos_t, float sin_t, void *stream)
+{
+    g_last_error.clear();
+    FL_REQUIRE(N >= 0 && H >= 1 && W >= 1 && H < (1 << 20) && W < (1 << 20), "shape: bad size");
+    FL_REQUIRE(N < (1 << 24), "shape: too many planes");
+    if (N == 0) return FL_OK;
+    FL_REQUIRE(x && out && x != out, "null or aliased pointer argument");
+    return launch_rotate(x, out, N, H, W, cos_t, sin_t, (hipStream_t)stream);
+}
+
+int fl_incident_atten(const float *frac, const float *coef, float *out, int64_t K, int64_t B, int64_t H, int64_t W,
+                      int64_t frac_element_stride, int64_t out_element_stride, void *stream)
+{
+    g_last_error.clear();
+    FL_REQUIRE(K >= 0 && B >= 0 && H >= 1 && W >= 1 && K < 65536 && B < 65536, "shape: bad size");
+    if (K == 0 || B == 0) return FL_OK;
+    FL_REQUIRE(frac && out, "null pointer argument");
+    return launch_incident(frac, coef, out, frac_element_stride, out_element_stride, K, B, H, W,
+                           (hipStream_t)stream);
+}
+
+int fl_mu_xrf_rot(const float *frac, const float *coef, float *mu_rot, int64_t Kc, int64_t K, int64_t B, int64_t H,
+                  int64_t W, int side, void *stream)
+{
+    g_last_error.clear();
+    FL_REQUIRE(Kc >= 1 && K >= 1 && B >= 0 && H >= 1 && W >= 1, "shape: bad size");
+    FL_REQUIRE(side == 0 || side == 1, "side must be 0 (detector at +col) or 1 (at -col)");
+    FL_REQUIRE(B * ((K + 3) / 4) < 65536 && H < (1 << 20), "shape: too large");
+    if (B == 0) return FL_OK;
+    FL_REQUIRE(frac && coef && mu_rot, "null pointer argument");
+    return launch_mu_rot(frac, coef, mu_rot, Kc, K, B, H, W, side, (hipStream_t)stream);
+}
+
+int fl_cone_atten(const float *mu_rot, float *T, int64_t K, int64_t Z, int64_t X, int64_t Y, const int32_t *pz,
+                  const int32_t *py, const int32_t *q, const float *weight, int64_t ndir, void *stream)
+{
+    g_last_error.clear();
+    FL_REQUIRE(K >= 1 && Z >= 1 && X >= 1 && Y >= 1 && ndir >= 1, "shape: bad size");
+    FL_REQUIRE(Z < (1 << 20) && X < (1 << 20) && Y < (1 << 20) && K < 2048, "shape: too large");
+    FL_REQUIRE(mu_rot && T && pz && py && q && weight, "null pointer argument");
+    FL_REQUIRE(mu_rot != T, "T must not alias mu");
+    for (int64_t d = 0; d < ndir; ++d)
+        if (int e = launch_cone_direction(mu_rot, T, K, Z, X, Y, pz[d], py[d], q[d], weight[d], d > 0,
+                                          (hipStream_t)stream))
+            return e;
+    return FL_OK;
+}
+
+int fl_xrf_combine(const float *T_rot, const float *incident, float *out, int64_t K, int64_t B, int64_t H,
+                   int64_t W, int64_t inc_element_stride, int64_t out_element_stride, int side, void *stream)
+{
+    g_last_error.clear();
+    FL_REQUIRE(K >= 1 && B >= 0 && H >= 1 && W >= 1 && B * K < 65536, "shape: bad size");
+    FL_REQUIRE(side == 0 || side == 1, "side must be 0 or 1");
+    if (B == 0) return FL_OK;
+    FL_REQUIRE(T_rot && out, "null pointer argument");
+    return launch_combine(T_rot, incident, out, inc_element_stride, out_element_stride, K, B, H, W, side,
+                          (hipStream_t)stream);
 }
 
 }  // extern "C"

@@ -9,52 +9,58 @@
 //   Pf[a,b,w] = sum_h atten[a,b,h,w] * sum_r em[a,r] * C_r(x, y)       (.cu:298-337)
 //   out[r,b]  = transpose of the same sampling, weighted by em*atten*ratio (.cu:390-446)
 //
-// Forward: one workgroup owns a 64x64 detector tile of one slice and a group of
-// angles.  The image-space footprint of the tile (the rotated tile's bounding
-// box, unioned over as many consecutive angles as fit) is staged in LDS once;
-// every bilinear tap is then an LDS read.  Each of the 4 waves sums 16 rows per
-// lane (lane = detector column) in registers; the 4 wave partials are summed in
-// a fixed order and written as one partial per 64-row tile.  A second small
-// kernel sums the ceil(H/64) partials in order.  No atomics: deterministic.
+// Forward: one workgroup owns a 64-column detector strip of one slice for a
+// group of 8 angles and keeps the 8 column sums of every lane (lane = detector
+// column) in registers while it walks the strip's full height in 32-row chunks.
+// For each chunk, the image footprint of the chunk (the rotated 64x32 tile's
+// bounding box, unioned over as many of the 8 angles as fit) is staged in LDS,
+// so every bilinear tap is an LDS read.  The 4 waves (8 rows each) are summed in
+// a fixed order at the end and written out directly, or turned into the MLEM
+// ratio I / max(Pf, 1e-6).  No atomics, no partial-sum buffer: deterministic.
 //
-// Back-projection: one wave owns a 32x64 image tile of one slice and keeps its
-// accumulator (plus a one-voxel halo) privately in LDS.  For every angle it
-// walks the detector points whose clamped sample lands in the tile, in 8x8
-// blocks (empty blocks skipped by a wave ballot), and splats the four bilinear
-// weights into its own accumulator.  No global atomics; nothing is shared
-// between waves, so the result does not depend on wave scheduling.  The MLEM
-// update can be fused into the write-out.
+// Back-projection: one workgroup owns a 32x64 image tile of one slice; each of
+// its 4 waves keeps a private LDS accumulator (tile plus a one-voxel halo).  For
+// every angle the waves split the rows of 8x8 detector-point blocks that cover
+// the tile's preimage (per block row, the column interval comes from the
+// rotated region's edges), load four blocks at a time (atten and ratio), and
+// splat the bilinear weights of the points that land in the tile into their
+// own accumulator.  No global atomics; the four accumulators are summed in a
+// fixed order at the end, which is also where the MLEM update is applied.
 #include "common.hpp"
 
 #include <algorithm>
 #include <climits>
-#include <vector>
 
 namespace fl {
 
 // ---------------------------------------------------------------------------
 // Forward projector
 // ---------------------------------------------------------------------------
-constexpr int kFTileW = 64;
-constexpr int kFTileH = 64;
+constexpr int kFW = 64;                   // detector columns per workgroup (one per lane)
+constexpr int kFH = 32;                   // detector rows per chunk
 constexpr int kFWaves = 4;
-constexpr int kFRowsPerWave = kFTileH / kFWaves;
-// Upper bound of one angle's staged footprint for a 64x64 tile: the rotated
-// tile spans at most 63*sqrt(2) < 89.1 rows/cols, plus the 0.5 margins and the
-// bilinear neighbour: <= 92 rows, pitch <= 93.
-constexpr int kFMaxFootprint = 93 * 93;
+constexpr int kFRows = kFH / kFWaves;     // rows per wave per chunk
+constexpr int kFG = 8;                    // angles per workgroup (one accumulator each)
+constexpr int kFLds = 11264;              // footprint floats staged per workgroup (44 KB)
+// One angle's footprint for a 64x32 tile: the sample points span at most
+// hypot(31, 63) < 70.3 rows (and cols); with the 0.5 margins and the bilinear
+// neighbour that is <= 74 rows and a pitch <= 75.
+constexpr int kFMaxFootprint = 74 * 75;
+constexpr int kFRefChunk = kFLds / kFMaxFootprint;  // references staged together (2)
+
+enum FwdMode { kFwdWrite = 0, kFwdAccumulate = 1, kFwdRatio = 2, kFwdAccumulateRatio = 3 };
 
 struct FwdArgs {
     const float *atten;
     const float *C;
     const float *em;
     const float *theta;
-    float *part;  // (nht, A, B, W)
+    const float *I;  // ratio modes
+    float *out;      // (A, B, W): Pf or ratio
     int A, B, R, H, W;
     int r0, nref;
-    int accumulate;
-    int nht, ntw, n_groups, group;
-    int lds_floats;
+    int mode;
+    int ntw, n_groups;
 };
 
 struct Box {
@@ -88,159 +94,161 @@ __device__ __forceinline__ Box fwd_footprint(float c, float s, int h0, int h1, i
     return b;
 }
 
-__global__ __launch_bounds__(256) void fwd_tile_kernel(FwdArgs p)
+__global__ __launch_bounds__(256) void fwd_strip_kernel(FwdArgs p)
 {
     extern __shared__ float lds[];
-    float *red = lds + p.lds_floats;  // [2][kFWaves][64]
+    __shared__ float trig[2 * kFG];
+    __shared__ float emv[kFG * kFRefChunk];
 
     uint32_t t = xcd_remap(blockIdx.x, gridDim.x);
     const int tw = t % p.ntw;
     t /= p.ntw;
-    const int th = t % p.nht;
-    t /= p.nht;
     const int g = t % p.n_groups;
     const int b = t / p.n_groups;
 
     const int lane = threadIdx.x & 63;
     const int wave = threadIdx.x >> 6;
-    const int w0 = tw * kFTileW, w1 = min(w0 + kFTileW, p.W);
-    const int h0 = th * kFTileH, h1 = min(h0 + kFTileH, p.H);
+    const int w0 = tw * kFW, w1 = min(w0 + kFW, p.W);
     const int w = w0 + lane;
     const bool wvalid = w < p.W;
     const float cx = (p.W - 1) * 0.5f, cy = (p.H - 1) * 0.5f;
     const float xmax = p.H - 1.001f, ymax = p.W - 1.001f;
     const int64_t HW = (int64_t)p.H * p.W;
-    const int a_begin = g * p.group, a_end = min(a_begin + p.group, p.A);
-    // Lanes past the last column sample the last valid column (their atten is 0).
+    const int a0 = g * kFG;
+    const int na = min(kFG, p.A - a0);
+    // Lanes past the last column sample the last valid column (their atten is 0),
+    // so every LDS index stays inside the staged footprint.
     const float wc = (float)min(w, w1 - 1) - cx;
-    const int hw0 = h0 + wave * kFRowsPerWave;
 
-    int a = a_begin;
-    while (a < a_end) {
-        // Union footprint of the longest run of angles that fits the LDS budget.
-        Box u;
-        int a_hi = a;
-        {
-            const float th0 = finite_or_zero(p.theta[a]);
-            u = fwd_footprint(cosf(-th0), sinf(-th0), h0, h1, w0, w1, cx, cy, xmax, ymax);
-            ++a_hi;
-            while (a_hi < a_end) {
-                const float tha = finite_or_zero(p.theta[a_hi]);
-                Box nb = fwd_footprint(cosf(-tha), sinf(-tha), h0, h1, w0, w1, cx, cy, xmax, ymax);
+    if (threadIdx.x < kFG) {
+        const int k = threadIdx.x;
+        const float th = k < na ? finite_or_zero(p.theta[a0 + k]) : 0.0f;
+        trig[2 * k] = cosf(-th);
+        trig[2 * k + 1] = sinf(-th);
+    }
+    if (threadIdx.x < kFG * kFRefChunk) {
+        const int k = threadIdx.x / kFRefChunk, rr = threadIdx.x % kFRefChunk;
+        emv[threadIdx.x] = (k < na && rr < p.nref) ? p.em[(int64_t)(a0 + k) * p.R + p.r0 + rr] : 0.0f;
+    }
+    __syncthreads();
+
+    float acc[kFG];
+#pragma unroll
+    for (int k = 0; k < kFG; ++k) acc[k] = 0.0f;
+
+    for (int h0 = 0; h0 < p.H; h0 += kFH) {
+        const int h1 = min(h0 + kFH, p.H);
+        const int hr0 = h0 + wave * kFRows;
+        int gi = 0;
+        while (gi < na) {
+            // Longest run of angles gi..gj-1 whose union footprint fits the budget.
+            Box u = fwd_footprint(trig[2 * gi], trig[2 * gi + 1], h0, h1, w0, w1, cx, cy, xmax, ymax);
+            int gj = gi + 1;
+            while (gj < na) {
+                Box nb = fwd_footprint(trig[2 * gj], trig[2 * gj + 1], h0, h1, w0, w1, cx, cy, xmax, ymax);
                 Box m{min(u.rlo, nb.rlo), max(u.rhi, nb.rhi), min(u.clo, nb.clo), max(u.chi, nb.chi)};
-                const int pitch = (m.chi - m.clo + 1) | 1;
-                if ((m.rhi - m.rlo + 1) * pitch * p.nref > p.lds_floats) break;
+                if ((m.rhi - m.rlo + 1) * ((m.chi - m.clo + 1) | 1) * p.nref > kFLds) break;
                 u = m;
-                ++a_hi;
+                ++gj;
             }
-        }
-        const int rlo = uniform_int(u.rlo), clo = uniform_int(u.clo);
-        const int rows = uniform_int(u.rhi) - rlo + 1;
-        const int cols = uniform_int(u.chi) - clo + 1;
-        const int pitch = cols | 1;
-        const int fsz = rows * pitch;
+            const int rlo = uniform_int(u.rlo), clo = uniform_int(u.clo);
+            const int rows = uniform_int(u.rhi) - rlo + 1;
+            const int cols = uniform_int(u.chi) - clo + 1;
+            const int pitch = cols | 1;
+            const int fsz = rows * pitch;
+            gj = uniform_int(gj);
 
-        __syncthreads();  // previous footprint and partial buffers are free
-        for (int rr = 0; rr < p.nref; ++rr) {
-            const float *Cs = p.C + ((int64_t)(p.r0 + rr) * p.B + b) * HW;
-            float *dst = lds + rr * fsz;
-            for (int r = wave; r < rows; r += kFWaves) {
-                const float *src = Cs + (int64_t)(rlo + r) * p.W + clo;
-                for (int cc = lane; cc < cols; cc += 64) dst[r * pitch + cc] = src[cc];
-            }
-        }
-        __syncthreads();
-
-        for (int aa = a; aa < a_hi; ++aa) {
-            const float tha = finite_or_zero(p.theta[aa]);
-            const float c = cosf(-tha), s = sinf(-tha);
-            const float bx = fmaf(-s, wc, cy);
-            const float by = fmaf(c, wc, cx);
-            const float *ap = p.atten + ((int64_t)aa * p.B + b) * HW + (int64_t)hw0 * p.W + w;
-            float av[kFRowsPerWave];
-#pragma unroll
-            for (int i = 0; i < kFRowsPerWave; ++i)
-                av[i] = (wvalid && hw0 + i < h1) ? ap[(int64_t)i * p.W] : 0.0f;
-            // Rows past the image edge sample the last valid row (times atten 0),
-            // so every LDS index stays inside the staged footprint.
-
-            float acc = 0.0f;
-            if (p.nref == 1) {
-                const float e = p.em[(int64_t)aa * p.R + p.r0];
-#pragma unroll
-                for (int i = 0; i < kFRowsPerWave; ++i) {
-                    const float hc = (float)min(hw0 + i, h1 - 1) - cy;
-                    const float x = clampf(fmaf(c, hc, bx), 0.f, xmax);
-                    const float y = clampf(fmaf(s, hc, by), 0.f, ymax);
-                    const int x0 = (int)x, y0 = (int)y;
-                    const float fx = x - (float)x0, fy = y - (float)y0;
-                    const int li = (x0 - rlo) * pitch + (y0 - clo);
-                    const float v00 = lds[li], v01 = lds[li + 1];
-                    const float v10 = lds[li + pitch], v11 = lds[li + pitch + 1];
-                    const float top = fmaf(fy, v01 - v00, v00);
-                    const float bot = fmaf(fy, v11 - v10, v10);
-                    const float val = fmaf(fx, bot - top, top);
-                    acc = fmaf(val, av[i], acc);
+            __syncthreads();  // everyone is done with the previous footprint
+            for (int rr = 0; rr < p.nref; ++rr) {
+                const float *Cs = p.C + ((int64_t)(p.r0 + rr) * p.B + b) * HW;
+                float *dst = lds + rr * fsz;
+                for (int r = wave; r < rows; r += kFWaves) {
+                    const float *src = Cs + (int64_t)(rlo + r) * p.W + clo;
+                    for (int cc = lane; cc < cols; cc += 64) dst[r * pitch + cc] = src[cc];
                 }
-                acc *= e;
-            } else {
+            }
+            __syncthreads();
+
 #pragma unroll
-                for (int i = 0; i < kFRowsPerWave; ++i) {
-                    const float hc = (float)min(hw0 + i, h1 - 1) - cy;
+            for (int k = 0; k < kFG; ++k) {
+                if (k < gi || k >= gj) continue;  // wave-uniform
+                const float c = trig[2 * k], s = trig[2 * k + 1];
+                const float bx = fmaf(-s, wc, cy);
+                const float by = fmaf(c, wc, cx);
+                const float *ap = p.atten + ((int64_t)(a0 + k) * p.B + b) * HW + (int64_t)hr0 * p.W + w;
+                float av[kFRows];
+#pragma unroll
+                for (int i = 0; i < kFRows; ++i)
+                    av[i] = (wvalid && hr0 + i < h1) ? __builtin_nontemporal_load(ap + (int64_t)i * p.W) : 0.0f;
+                float sum = 0.0f;
+#pragma unroll
+                for (int i = 0; i < kFRows; ++i) {
+                    // Rows past the image edge sample the last valid row (times atten 0).
+                    const float hc = (float)min(hr0 + i, h1 - 1) - cy;
                     const float x = clampf(fmaf(c, hc, bx), 0.f, xmax);
                     const float y = clampf(fmaf(s, hc, by), 0.f, ymax);
                     const int x0 = (int)x, y0 = (int)y;
                     const float fx = x - (float)x0, fy = y - (float)y0;
                     const int li = (x0 - rlo) * pitch + (y0 - clo);
-                    float sum = 0.0f;
-                    for (int rr = 0; rr < p.nref; ++rr) {
-                        const float *f = lds + rr * fsz;
-                        const float v00 = f[li], v01 = f[li + 1];
-                        const float v10 = f[li + pitch], v11 = f[li + pitch + 1];
+                    if (p.nref == 1) {
+                        const float v00 = lds[li], v01 = lds[li + 1];
+                        const float v10 = lds[li + pitch], v11 = lds[li + pitch + 1];
                         const float top = fmaf(fy, v01 - v00, v00);
                         const float bot = fmaf(fy, v11 - v10, v10);
-                        sum = fmaf(p.em[(int64_t)aa * p.R + p.r0 + rr], fmaf(fx, bot - top, top), sum);
+                        sum = fmaf(fmaf(fx, bot - top, top), av[i], sum);
+                    } else {
+                        float val = 0.0f;
+                        for (int rr = 0; rr < p.nref; ++rr) {
+                            const float *f = lds + rr * fsz;
+                            const float v00 = f[li], v01 = f[li + 1];
+                            const float v10 = f[li + pitch], v11 = f[li + pitch + 1];
+                            const float top = fmaf(fy, v01 - v00, v00);
+                            const float bot = fmaf(fy, v11 - v10, v10);
+                            val = fmaf(emv[k * kFRefChunk + rr], fmaf(fx, bot - top, top), val);
+                        }
+                        sum = fmaf(val, av[i], sum);
                     }
-                    acc = fmaf(sum, av[i], acc);
                 }
+                acc[k] += sum;
             }
-            float *rb = red + (aa & 1) * (kFWaves * 64);
-            rb[wave * 64 + lane] = acc;
-            __syncthreads();
-            if (wave == 0 && wvalid) {
-                const float tot = ((rb[lane] + rb[64 + lane]) + rb[128 + lane]) + rb[192 + lane];
-                float *o = p.part + (((int64_t)th * p.A + aa) * p.B + b) * p.W + w;
-                *o = p.accumulate ? *o + tot : tot;
-            }
+            gi = gj;
         }
-        a = a_hi;
     }
-}
 
-// Pf[a,b,w] = sum_t part[t,a,b,w] in order t = 0..nht-1.
-// mode 0: write Pf.  mode 1: write ratio = I / max(Pf, 1e-6) (atten_cuda.cpp:326-328).
-__global__ __launch_bounds__(256) void fwd_reduce_kernel(const float *part, const float *I, float *out,
-                                                         int64_t n, int nht, int mode)
-{
-    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    float s = part[i];
-    for (int t = 1; t < nht; ++t) s += part[(int64_t)t * n + i];
-    if (mode == 1) {
-        const float pf = s < 1e-6f ? 1e-6f : s;
-        s = I[i] / pf;
+    // Fixed-order reduction of the four waves' partial sums.
+    __syncthreads();
+    float *red = lds;  // [kFG][kFWaves][64]
+#pragma unroll
+    for (int k = 0; k < kFG; ++k) red[(k * kFWaves + wave) * 64 + lane] = acc[k];
+    __syncthreads();
+    for (int idx = threadIdx.x; idx < na * 64; idx += 256) {
+        const int k = idx >> 6, l = idx & 63;
+        const int wo = w0 + l;
+        if (wo >= p.W) continue;
+        const float *rk = red + k * kFWaves * 64 + l;
+        float tot = ((rk[0] + rk[64]) + rk[128]) + rk[192];
+        if (p.nref == 1) tot *= emv[k * kFRefChunk];
+        const int64_t o = ((int64_t)(a0 + k) * p.B + b) * p.W + wo;
+        if (p.mode == kFwdAccumulate || p.mode == kFwdAccumulateRatio) tot += p.out[o];
+        if (p.mode >= kFwdRatio) {
+            // Pf.clamp_min_(1e-6); ratio = I / Pf   (atten_cuda.cpp:326-328)
+            const float pf = tot < 1e-6f ? 1e-6f : tot;
+            tot = p.I[o] / pf;
+        }
+        p.out[o] = tot;
     }
-    out[i] = s;
 }
 
 // ---------------------------------------------------------------------------
 // Back-projection (exact transpose) and the fused MLEM update
 // ---------------------------------------------------------------------------
-constexpr int kBTileI = 32;  // image rows per wave tile
-constexpr int kBTileJ = 64;  // image cols per wave tile
-constexpr int kBWaves = 4;   // one slice per wave
-constexpr int kBPitch = kBTileJ + 3;  // halo of one on each side, odd pitch
-constexpr int kBAcc = (kBTileI + 2) * kBPitch;
+constexpr int kBI = 32;               // image rows per tile
+constexpr int kBJ = 64;               // image cols per tile
+constexpr int kBWaves = 4;            // private accumulators per workgroup
+constexpr int kBPitch = kBJ + 3;      // halo of one on each side, odd pitch
+constexpr int kBAcc = (kBI + 2) * kBPitch;
+constexpr int kBRefChunk = 2;
+constexpr int kBUnroll = 4;           // 8x8 blocks loaded together
 
 enum BwdMode { kBwdWrite = 0, kBwdSens = 1, kBwdUpdate = 2 };
 
@@ -276,6 +284,24 @@ __device__ __forceinline__ float huber_at(const float *Cs, int i, int j, int H, 
     return g;
 }
 
+// Range of qc with lo <= slope*qc + base(h) < hi for some h of a block row, where
+// base(h) runs between bmin and bmax over the row's h.  Conservative by one.
+__device__ __forceinline__ void slope_interval(float slope, float lo, float hi, float bmin, float bmax,
+                                               float &qmin, float &qmax)
+{
+    const float L = lo - bmax, U = hi - bmin;  // need L <= slope*qc < U
+    if (slope > 1e-6f) {
+        qmin = fmaxf(qmin, L / slope - 1.0f);
+        qmax = fminf(qmax, U / slope + 1.0f);
+    } else if (slope < -1e-6f) {
+        qmin = fmaxf(qmin, U / slope - 1.0f);
+        qmax = fminf(qmax, L / slope + 1.0f);
+    } else if (!(L <= 1e-3f && U > -1e-3f)) {
+        qmin = 1.0f;
+        qmax = 0.0f;  // empty
+    }
+}
+
 __global__ __launch_bounds__(256) void bwd_tile_kernel(BwdArgs p)
 {
     extern __shared__ float lds[];
@@ -283,13 +309,11 @@ __global__ __launch_bounds__(256) void bwd_tile_kernel(BwdArgs p)
     const int tj = t % p.ntj;
     t /= p.ntj;
     const int ti = t % p.nti;
-    const int bg = t / p.nti;
+    const int b = t / p.nti;
 
     const int lane = threadIdx.x & 63;
     const int wave = threadIdx.x >> 6;
-    const int b = bg * kBWaves + wave;
-    const bool active = b < p.B;
-    const int I0 = ti * kBTileI, J0 = tj * kBTileJ;
+    const int I0 = ti * kBI, J0 = tj * kBJ;
     float *acc = lds + wave * p.nref * kBAcc;
     for (int k = lane; k < p.nref * kBAcc; k += 64) acc[k] = 0.0f;
 
@@ -297,20 +321,23 @@ __global__ __launch_bounds__(256) void bwd_tile_kernel(BwdArgs p)
     const float xmax = p.H - 1.001f, ymax = p.W - 1.001f;
     const int64_t HW = (int64_t)p.H * p.W;
     constexpr float BIG = 1.0e6f;
-    // Clamped sample rows landing in [I0-1, I0+T): unclamped bounds, open where
-    // the clamp folds the outside onto the tile.
+    // Unclamped sample rows whose clamped value lands in [I0-1, I0+T): open on a
+    // side where the clamp folds everything beyond onto the tile.
     const float lox = I0 <= 1 ? -BIG : (float)(I0 - 1);
-    const float hix = I0 + kBTileI >= p.H - 1 ? BIG : (float)(I0 + kBTileI);
+    const float hix = I0 + kBI >= p.H - 1 ? BIG : (float)(I0 + kBI);
     const float loy = J0 <= 1 ? -BIG : (float)(J0 - 1);
-    const float hiy = J0 + kBTileJ >= p.W - 1 ? BIG : (float)(J0 + kBTileJ);
+    const float hiy = J0 + kBJ >= p.W - 1 ? BIG : (float)(J0 + kBJ);
+    const bool border = lox < 0.0f || hix > (float)p.H || loy < 0.0f || hiy > (float)p.W;
     const int lh = lane >> 3, lw = lane & 7;
 
-    if (active) {
-        for (int a = 0; a < p.A; ++a) {
-            const float tha = finite_or_zero(p.theta[a]);
-            const float c = cosf(tha), s = sinf(tha);
-            // Preimage bounding box: invert the rotation at the region corners.
-            float hmin = 3.0e38f, hmax = -3.0e38f, qmin = 3.0e38f, qmax = -3.0e38f;
+    for (int a = 0; a < p.A; ++a) {
+        const float tha = finite_or_zero(p.theta[a]);
+        const float c = cosf(tha), s = sinf(tha);
+        // backward_emission_batch_kernel (.cu:416-417):
+        //   col = c*(q-cx) - s*(h-cy) + cx,  row = s*(q-cx) + c*(h-cy) + cy
+        // Preimage bounding box: invert the rotation at the region's corners.
+        float hmin = 3.0e38f, hmax = -3.0e38f;
+        {
             const float Xs[2] = {lox - cy, hix - cy};
             const float Ys[2] = {loy - cx, hiy - cx};
 #pragma unroll
@@ -318,69 +345,82 @@ __global__ __launch_bounds__(256) void bwd_tile_kernel(BwdArgs p)
 #pragma unroll
                 for (int j = 0; j < 2; ++j) {
                     const float hc = c * Xs[i] - s * Ys[j];
-                    const float qc = s * Xs[i] + c * Ys[j];
                     hmin = fminf(hmin, hc);
                     hmax = fmaxf(hmax, hc);
-                    qmin = fminf(qmin, qc);
-                    qmax = fmaxf(qmax, qc);
                 }
-            const int hlo = uniform_int((int)fmaxf(floorf(hmin + cy) - 1.f, 0.f));
-            const int hhi = uniform_int((int)fminf(ceilf(hmax + cy) + 1.f, (float)(p.H - 1)));
-            const int qlo = uniform_int((int)fmaxf(floorf(qmin + cx) - 1.f, 0.f));
-            const int qhi = uniform_int((int)fminf(ceilf(qmax + cx) + 1.f, (float)(p.W - 1)));
-            const float *ap = p.atten + ((int64_t)a * p.B + b) * HW;
-            const float *rp = p.ratio ? p.ratio + ((int64_t)a * p.B + b) * p.W : nullptr;
-            const float *ep = p.em + (int64_t)a * p.R + p.r0;
+        }
+        const int hlo = uniform_int((int)clampf(floorf(hmin + cy) - 1.f, 0.f, (float)p.H));
+        const int hhi = uniform_int((int)clampf(ceilf(hmax + cy) + 1.f, -1.f, (float)(p.H - 1)));
+        const float *ap = p.atten + ((int64_t)a * p.B + b) * HW;
+        const float *rp = p.ratio ? p.ratio + ((int64_t)a * p.B + b) * p.W : nullptr;
+        float ew[kBRefChunk];
+#pragma unroll
+        for (int rr = 0; rr < kBRefChunk; ++rr) ew[rr] = rr < p.nref ? p.em[(int64_t)a * p.R + p.r0 + rr] : 0.0f;
 
-            for (int hb = hlo; hb <= hhi; hb += 8) {
-                const int h = hb + lh;
-                const float hc = (float)h - cy;
-                for (int qb = qlo; qb <= qhi; qb += 8) {
-                    const int q = qb + lw;
+        for (int hb = hlo + 8 * wave; hb <= hhi; hb += 8 * kBWaves) {
+            // Column interval of this block row: rows hb..hb+7.
+            const float hcA = (float)hb - cy, hcB = (float)(hb + 7) - cy;
+            float qmin = -cx - 1.0f, qmax = (float)(p.W - 1) - cx + 1.0f;
+            slope_interval(s, lox - cy, hix - cy, fminf(c * hcA, c * hcB), fmaxf(c * hcA, c * hcB), qmin, qmax);
+            slope_interval(c, loy - cx, hiy - cx, fminf(-s * hcA, -s * hcB), fmaxf(-s * hcA, -s * hcB), qmin, qmax);
+            const int ql = uniform_int((int)clampf(floorf(qmin + cx), 0.f, (float)p.W));
+            const int qh = uniform_int((int)clampf(ceilf(qmax + cx), -1.f, (float)(p.W - 1)));
+            if (ql > qh) continue;
+            const int h = hb + lh;
+            const float hc = (float)h - cy;
+            const float rowb = fmaf(c, hc, cy);
+            const float colb = fmaf(-s, hc, cx);
+            for (int qb = ql; qb <= qh; qb += 8 * kBUnroll) {
+                float av[kBUnroll], rv[kBUnroll], X[kBUnroll], Y[kBUnroll];
+                bool in[kBUnroll];
+#pragma unroll
+                for (int u = 0; u < kBUnroll; ++u) {
+                    const int q = qb + 8 * u + lw;
                     const float qc = (float)q - cx;
-                    // backward_emission_batch_kernel (.cu:416-421)
-                    const float col_u = c * qc - s * hc + cx;
-                    const float row_u = s * qc + c * hc + cy;
-                    const bool in = h <= hhi && q <= qhi && row_u >= lox && row_u < hix &&
-                                    col_u >= loy && col_u < hiy;
-                    if (__builtin_amdgcn_ballot_w64(in) == 0) continue;
-                    if (in) {
-                        const float rc = clampf(row_u, 0.f, xmax);
-                        const float cc = clampf(col_u, 0.f, ymax);
-                        const int y0 = (int)rc, x0 = (int)cc;
-                        const float dy = rc - (float)y0, dx = cc - (float)x0;
-                        float v = ap[(int64_t)h * p.W + q];
-                        if (rp) v *= rp[q];
-                        const int li = (y0 - I0 + 1) * kBPitch + (x0 - J0 + 1);
-                        const float a0 = 1.0f - dy, a1 = dy;
-                        const float b0 = 1.0f - dx, b1 = dx;
-                        for (int rr = 0; rr < p.nref; ++rr) {
-                            const float wv = ep[rr] * v;
-                            float *A_ = acc + rr * kBAcc + li;
-                            const float wt = wv * a0, wb = wv * a1;
-                            atomicAdd(A_, wt * b0);
-                            atomicAdd(A_ + 1, wt * b1);
-                            atomicAdd(A_ + kBPitch, wb * b0);
-                            atomicAdd(A_ + kBPitch + 1, wb * b1);
-                        }
+                    X[u] = fmaf(s, qc, rowb);
+                    Y[u] = fmaf(c, qc, colb);
+                    in[u] = h <= hhi && q <= qh && X[u] >= lox && X[u] < hix && Y[u] >= loy && Y[u] < hiy;
+                    av[u] = in[u] ? ap[(int64_t)h * p.W + q] : 0.0f;
+                    rv[u] = (in[u] && rp) ? rp[q] : 1.0f;
+                }
+#pragma unroll
+                for (int u = 0; u < kBUnroll; ++u) {
+                    if (!in[u]) continue;
+                    const float rc = border ? clampf(X[u], 0.f, xmax) : X[u];
+                    const float cc = border ? clampf(Y[u], 0.f, ymax) : Y[u];
+                    const int y0 = (int)rc, x0 = (int)cc;
+                    const float dy = rc - (float)y0, dx = cc - (float)x0;
+                    const float v = av[u] * rv[u];
+                    const int li = (y0 - I0 + 1) * kBPitch + (x0 - J0 + 1);
+                    const float b0 = 1.0f - dx, b1 = dx;
+#pragma unroll
+                    for (int rr = 0; rr < kBRefChunk; ++rr) {
+                        if (rr >= p.nref) break;
+                        const float wv = ew[rr] * v;
+                        const float wt = wv * (1.0f - dy), wb = wv * dy;
+                        float *A_ = acc + rr * kBAcc + li;
+                        atomicAdd(A_, wt * b0);
+                        atomicAdd(A_ + 1, wt * b1);
+                        atomicAdd(A_ + kBPitch, wb * b0);
+                        atomicAdd(A_ + kBPitch + 1, wb * b1);
                     }
                 }
             }
         }
     }
     __syncthreads();
-    if (!active) return;
 
-    // Write-out of the tile interior (lane = column).
+    // Sum the four private accumulators in a fixed order and write the tile (lane = column).
     const int j = J0 + lane;
     for (int rr = 0; rr < p.nref; ++rr) {
         const int r = p.r0 + rr;
         const int64_t sb = ((int64_t)r * p.B + b) * HW;
-        const float *A_ = acc + rr * kBAcc;
-        for (int ii = 0; ii < kBTileI; ++ii) {
+        for (int ii = wave; ii < kBI; ii += kBWaves) {
             const int i = I0 + ii;
-            if (i >= p.H || j >= p.W || lane >= kBTileJ) continue;
-            const float back = A_[(ii + 1) * kBPitch + lane + 1];
+            if (i >= p.H || j >= p.W) continue;
+            const int li = rr * kBAcc + (ii + 1) * kBPitch + lane + 1;
+            const int st = p.nref * kBAcc;
+            const float back = ((lds[li] + lds[st + li]) + lds[2 * st + li]) + lds[3 * st + li];
             const int64_t o = sb + (int64_t)i * p.W + j;
             if (p.mode == kBwdWrite) {
                 p.out[o] = back;
@@ -412,52 +452,44 @@ __global__ __launch_bounds__(256) void huber_kernel(const float *C, float *grad,
 // ---------------------------------------------------------------------------
 // Host launchers
 // ---------------------------------------------------------------------------
-static int fwd_lds_floats() { return 10240; }  // 40 KB of footprint per workgroup
 
-int launch_forward(const float *atten, const float *C, const float *em, const float *theta,
-                   float *part, int64_t A, int64_t B, int64_t R, int64_t H, int64_t W,
-                   hipStream_t stream, int group)
+// mode: kFwdWrite (Pf) or kFwdRatio (ratio = I / max(Pf, 1e-6))
+int launch_forward(const float *atten, const float *C, const float *em, const float *theta, const float *I,
+                   float *out, int64_t A, int64_t B, int64_t R, int64_t H, int64_t W, int mode, hipStream_t stream)
 {
-    const int lds_floats = fwd_lds_floats();
-    const int ref_chunk = std::max(1, lds_floats / kFMaxFootprint);
+    static bool attr_set = false;
+    const size_t shmem = sizeof(float) * kFLds;
+    if (!attr_set) {
+        FL_HIP_CHECK(hipFuncSetAttribute((const void *)fwd_strip_kernel, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                         (int)shmem));
+        attr_set = true;
+    }
     FwdArgs p{};
     p.atten = atten;
     p.C = C;
     p.em = em;
     p.theta = theta;
-    p.part = part;
+    p.I = I;
+    p.out = out;
     p.A = (int)A;
     p.B = (int)B;
     p.R = (int)R;
     p.H = (int)H;
     p.W = (int)W;
-    p.nht = (int)((H + kFTileH - 1) / kFTileH);
-    p.ntw = (int)((W + kFTileW - 1) / kFTileW);
-    p.group = group;
-    p.n_groups = (int)((A + group - 1) / group);
-    p.lds_floats = lds_floats;
-    const int64_t nwg = (int64_t)p.ntw * p.nht * p.n_groups * B;
+    p.ntw = (int)((W + kFW - 1) / kFW);
+    p.n_groups = (int)((A + kFG - 1) / kFG);
+    const int64_t nwg = (int64_t)p.ntw * p.n_groups * B;
     FL_REQUIRE(nwg < (int64_t)UINT32_MAX, "forward projector: grid too large");
-    const size_t shmem = sizeof(float) * (lds_floats + 2 * kFWaves * 64);
-    for (int64_t r0 = 0; r0 < R; r0 += ref_chunk) {
+    for (int64_t r0 = 0; r0 < R; r0 += kFRefChunk) {
         p.r0 = (int)r0;
-        p.nref = (int)std::min<int64_t>(ref_chunk, R - r0);
-        p.accumulate = r0 > 0;
-        hipLaunchKernelGGL(fwd_tile_kernel, dim3((uint32_t)nwg), dim3(256), shmem, stream, p);
+        p.nref = (int)std::min<int64_t>(kFRefChunk, R - r0);
+        const bool first = r0 == 0, last = r0 + kFRefChunk >= R;
+        const bool ratio = mode == kFwdRatio;
+        p.mode = first ? (last && ratio ? kFwdRatio : kFwdWrite)
+                       : (last && ratio ? kFwdAccumulateRatio : kFwdAccumulate);
+        hipLaunchKernelGGL(fwd_strip_kernel, dim3((uint32_t)nwg), dim3(256), shmem, stream, p);
         FL_LAUNCH_CHECK();
     }
-    return FL_OK;
-}
-
-int launch_forward_reduce(const float *part, const float *I, float *out, int64_t A, int64_t B,
-                          int64_t H, int64_t W, int mode, hipStream_t stream)
-{
-    const int64_t n = A * B * W;
-    const int nht = (int)((H + kFTileH - 1) / kFTileH);
-    if (n == 0) return FL_OK;
-    hipLaunchKernelGGL(fwd_reduce_kernel, dim3((uint32_t)((n + 255) / 256)), dim3(256), 0, stream, part,
-                       I, out, n, nht, mode);
-    FL_LAUNCH_CHECK();
     return FL_OK;
 }
 
@@ -465,7 +497,12 @@ int launch_backward(const float *atten, const float *ratio, const float *em, con
                     float *out, int64_t A, int64_t B, int64_t R, int64_t H, int64_t W, int mode,
                     const float *C, const float *sens, float beta, float delta, hipStream_t stream)
 {
-    const int ref_chunk = 2;
+    static bool attr_set = false;
+    if (!attr_set) {
+        FL_HIP_CHECK(hipFuncSetAttribute((const void *)bwd_tile_kernel, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                         (int)(sizeof(float) * kBWaves * kBRefChunk * kBAcc)));
+        attr_set = true;
+    }
     BwdArgs p{};
     p.atten = atten;
     p.ratio = ratio;
@@ -477,25 +514,18 @@ int launch_backward(const float *atten, const float *ratio, const float *em, con
     p.R = (int)R;
     p.H = (int)H;
     p.W = (int)W;
-    p.nti = (int)((H + kBTileI - 1) / kBTileI);
-    p.ntj = (int)((W + kBTileJ - 1) / kBTileJ);
+    p.nti = (int)((H + kBI - 1) / kBI);
+    p.ntj = (int)((W + kBJ - 1) / kBJ);
     p.mode = mode;
     p.C = C;
     p.sens = sens;
     p.beta = beta;
     p.delta = delta;
-    const int64_t nwg = (int64_t)p.nti * p.ntj * ((B + kBWaves - 1) / kBWaves);
+    const int64_t nwg = (int64_t)p.nti * p.ntj * B;
     FL_REQUIRE(nwg < (int64_t)UINT32_MAX, "back-projector: grid too large");
-    static bool attr_set = false;
-    if (!attr_set) {
-        FL_HIP_CHECK(hipFuncSetAttribute((const void *)bwd_tile_kernel,
-                                         hipFuncAttributeMaxDynamicSharedMemorySize,
-                                         (int)(sizeof(float) * kBWaves * ref_chunk * kBAcc)));
-        attr_set = true;
-    }
-    for (int64_t r0 = 0; r0 < R; r0 += ref_chunk) {
+    for (int64_t r0 = 0; r0 < R; r0 += kBRefChunk) {
         p.r0 = (int)r0;
-        p.nref = (int)std::min<int64_t>(ref_chunk, R - r0);
+        p.nref = (int)std::min<int64_t>(kBRefChunk, R - r0);
         const size_t shmem = sizeof(float) * (size_t)kBWaves * p.nref * kBAcc;
         hipLaunchKernelGGL(bwd_tile_kernel, dim3((uint32_t)nwg), dim3(256), shmem, stream, p);
         FL_LAUNCH_CHECK();
@@ -512,11 +542,6 @@ int launch_huber(const float *C, float *grad, int64_t n_slices, int64_t H, int64
                        n_slices, (int)H, (int)W, delta);
     FL_LAUNCH_CHECK();
     return FL_OK;
-}
-
-size_t forward_part_floats(int64_t A, int64_t B, int64_t H, int64_t W)
-{
-    return (size_t)((H + kFTileH - 1) / kFTileH) * A * B * W;
 }
 
 }  // namespace fl

@@ -42,8 +42,35 @@ def _same_device(*ts):
     return d
 
 
-def _workspace(nbytes, device):
+def _workspace(nbytes, device):  # noqa: D401  (scratch for the MLEM entry point)
     return torch.empty((max(int(nbytes), 1) + 3) // 4, dtype=torch.float32, device=device)
+
+
+# ---------------------------------------------------------------------------
+# Detector-mask correlation: atten_cuda.cpp:12-28
+# ---------------------------------------------------------------------------
+
+
+def cal_atten_cuda(mu, mask):
+    """out[k,z,x,y] = sum_{dz,dx,dy} mask[dz,dx,dy] * mu[k, z+dz-Mz/2, x+dx, y+dy-My/2] with
+    out-of-range taps skipped (atten_cuda_kernel.cu:10-57).  mu (K,Z,X,Y), mask (Mz,Mx,My)."""
+    _check(isinstance(mu, torch.Tensor) and mu.is_cuda, "mu must be a CUDA tensor")
+    _check(isinstance(mask, torch.Tensor) and mask.is_cuda, "mask must be a CUDA tensor")
+    _check(mu.dtype == torch.float32, "mu must be float32")
+    _check(mask.dtype == torch.float32, "mask must be float32")
+    _check(mu.dim() == 4, "mu must be (K, Z, X, Y)")
+    _check(mask.dim() == 3, "mask must be (Mz, Mx, My)")
+    mu, mask = mu.contiguous(), mask.contiguous()
+    dev = _same_device(mu, mask)
+    out = torch.zeros_like(mu)
+    K, Z, X, Y = mu.shape
+    Mz, Mx, My = mask.shape
+    if mu.numel() == 0:
+        return out
+    with torch.cuda.device(dev):
+        _lib.call("fl_cal_atten", _lib.ptr(mu), _lib.ptr(mask), _lib.ptr(out), K, Z, X, Y, Mz, Mx, My,
+                  _lib.stream_of(dev))
+    return out
 
 
 # ---------------------------------------------------------------------------
@@ -97,10 +124,8 @@ def forward_emission_batch(atten, C, em_cs, theta):
     if A == 0 or B == 0 or W == 0:
         return Pf.zero_()
     with torch.cuda.device(dev):
-        nbytes = _lib.load().fl_projector_workspace_bytes(A, B, H, W)
-        ws = _workspace(nbytes, dev)
         _lib.call("fl_forward_emission_batch", _lib.ptr(atten), _lib.ptr(C), _lib.ptr(em_cs), _lib.ptr(theta),
-                  _lib.ptr(Pf), A, B, R, H, W, _lib.ptr(ws), ws.numel() * 4, _lib.stream_of(dev))
+                  _lib.ptr(Pf), A, B, R, H, W, _lib.stream_of(dev))
     return Pf
 
 

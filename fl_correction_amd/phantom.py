@@ -67,3 +67,28 @@ def poisson_noise(Pf, peak=1.0e4, seed=0):
     m = float(Pf.max())
     s = peak / m if m > 0 else 1.0
     return (rng.poisson(np.clip(Pf, 0, None) * s) / s).astype(np.float32)
+
+
+def ellipsoid_phantom_device(shape, n_elem, n_ell=(6, 12), semi_axes=(4.0, 16.0), conc=(0.1, 1.0), seed=0,
+                             device="cuda"):
+    """Same draws as ellipsoid_phantom, evaluated with torch on `device` (for large volumes)."""
+    import torch
+    B, H, W = shape
+    rng = rng_for(seed)
+    out = torch.zeros((n_elem, B, H, W), dtype=torch.float32, device=device)
+    zz = torch.arange(B, dtype=torch.float64, device=device)[:, None, None]
+    yy = torch.arange(H, dtype=torch.float64, device=device)[None, :, None]
+    xx = torch.arange(W, dtype=torch.float64, device=device)[None, None, :]
+    cy, cx = (H - 1) / 2.0, (W - 1) / 2.0
+    r_cyl = min(H, W) / 2.0
+    for e in range(n_elem):
+        n = int(rng.integers(n_ell[0], n_ell[1] + 1))
+        for _ in range(n):
+            ax = rng.uniform(semi_axes[0], semi_axes[1], size=3)
+            rad = r_cyl * 0.75 * np.sqrt(rng.uniform(0.0, 1.0))
+            phi = rng.uniform(0.0, 2 * np.pi)
+            c = (rng.uniform(0, B - 1), cy + rad * np.sin(phi), cx + rad * np.cos(phi))
+            val = float(np.float32(rng.uniform(conc[0], conc[1])))
+            inside = (((zz - c[0]) / ax[0]) ** 2 + ((yy - c[1]) / ax[1]) ** 2 + ((xx - c[2]) / ax[2]) ** 2) <= 1.0
+            out[e] += inside.to(torch.float32) * val
+    return out

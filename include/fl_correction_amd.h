@@ -49,18 +49,17 @@ FL_API int fl_version(void);
 /* Pf[a,b,w] = sum_h atten[a,b,h,w] * sum_r em_cs[a,r] * bilin_clamp(C[r,b], p_a(h,w)).
  * Replaces forward_emission_batch (FL/cuda_lib/atten_cuda.cpp:142-182;
  * kernel FL/cuda_lib/atten_cuda_kernel.cu:298-337).  atten (A,B,H,W),
- * C (R,B,H,W), em_cs (A,R), theta (A) radians, Pf (A,B,W).
- * `workspace` holds ceil(H/64)*A*B*W floats of partial sums
- * (fl_projector_workspace_bytes); the call allocates nothing. */
+ * C (R,B,H,W), em_cs (A,R), theta (A) radians, Pf (A,B,W).  Deterministic:
+ * every column sum is formed in a fixed order, without atomics. */
 FL_API int fl_forward_emission_batch(const float *atten, const float *C, const float *em_cs,
-                              const float *theta, float *Pf, int64_t A, int64_t B, int64_t R,
-                              int64_t H, int64_t W, void *workspace, size_t workspace_bytes,
-                              void *stream);
+                                     const float *theta, float *Pf, int64_t A, int64_t B, int64_t R,
+                                     int64_t H, int64_t W, void *stream);
 
 /* out[r,b] = sum_a em_cs[a,r] * S_a^T( atten[a,b] * ratio[a,b,:] ), S_a^T the exact
  * transpose of the bilinear sampling above.  Replaces backward_emission_batch
  * (atten_cuda.cpp:188-214; kernel atten_cuda_kernel.cu:390-446).  ratio (A,B,W),
- * out (R,B,H,W).  ratio == NULL means ratio == 1 (the MLEM sensitivity). */
+ * out (R,B,H,W).  ratio == NULL means ratio == 1 (the MLEM sensitivity).
+ * Gather-by-tile: no global atomics; per-tile sums in a fixed order. */
 FL_API int fl_backward_emission_batch(const float *atten, const float *ratio, const float *em_cs,
                                const float *theta, float *out, int64_t A, int64_t B, int64_t R,
                                int64_t H, int64_t W, void *stream);
@@ -69,14 +68,12 @@ FL_API int fl_backward_emission_batch(const float *atten, const float *ratio, co
  * (atten_cuda.cpp:82-108) and backward_emission (atten_cuda.cpp:113-135):
  * atten (A,H,W), C (R,H,W), Pf (A,W), ratio (A,W), out (R,H,W). */
 FL_API int fl_forward_emission(const float *atten, const float *C, const float *em_cs,
-                        const float *theta, float *Pf, int64_t A, int64_t R, int64_t H,
-                        int64_t W, void *workspace, size_t workspace_bytes, void *stream);
+                               const float *theta, float *Pf, int64_t A, int64_t R, int64_t H,
+                               int64_t W, void *stream);
 FL_API int fl_backward_emission(const float *atten, const float *ratio, const float *em_cs,
                          const float *theta, float *out, int64_t A, int64_t R, int64_t H,
                          int64_t W, void *stream);
 
-/* Bytes of workspace fl_forward_emission[_batch] needs. */
-FL_API size_t fl_projector_workspace_bytes(int64_t A, int64_t B, int64_t H, int64_t W);
 
 /* ---------------------------------------------------------------------------
  * MLEM / One-Step-Late Huber MAP-EM.
@@ -99,6 +96,82 @@ FL_API int fl_mlem_batch(const float *C_init, const float *atten, const float *e
                   int64_t R, int64_t H, int64_t W, int64_t n_iter, double beta, double delta,
                   void *workspace, size_t workspace_bytes, void *stream);
 FL_API size_t fl_mlem_workspace_bytes(int64_t A, int64_t B, int64_t R, int64_t H, int64_t W);
+
+/* The three stages of fl_mlem_batch, for callers that drive the iteration
+ * themselves (and time its kernels):
+ *   fl_mlem_sensitivity: sens = max(B^T 1, 1e-6)                  (atten_cuda.cpp:308-314)
+ *   fl_mlem_ratio:       ratio = I / max(F C, 1e-6), ratio (A,B,W)  (atten_cuda.cpp:323-328)
+ *   fl_mlem_update:      C_next = max(0, C*(B^T ratio)/(sens + beta*huber(C))), or
+ *                        C*(B^T ratio)/sens when beta <= 0         (atten_cuda.cpp:330-341);
+ *                        C_next must not alias C. */
+FL_API int fl_mlem_sensitivity(const float *atten, const float *em_cs, const float *theta,
+                               float *sens, int64_t A, int64_t B, int64_t R, int64_t H, int64_t W,
+                               void *stream);
+FL_API int fl_mlem_ratio(const float *atten, const float *C, const float *em_cs,
+                         const float *theta, const float *I, float *ratio, int64_t A, int64_t B,
+                         int64_t R, int64_t H, int64_t W, void *stream);
+FL_API int fl_mlem_update(const float *atten, const float *ratio, const float *em_cs,
+                          const float *theta, const float *C, const float *sens, float *C_next,
+                          int64_t A, int64_t B, int64_t R, int64_t H, int64_t W, double beta,
+                          double delta, void *stream);
+
+/* ---------------------------------------------------------------------------
+ * Attenuation maps.
+ *
+ * Frames: the original frame is (B, H, W) = (slice, row, col), incident beam
+ * along rows toward row 0, detector at +col (side 0) or -col (side 1).  The
+ * mask frame (Z, X, Y) is the frame atten_kernel works in: for side 0,
+ * X = col and Y = H-1-row (torch_rot90_4D 'clock', FL/torch_functions.py:143);
+ * for side 1, X = W-1-col and Y = row ('c-clock', :145).
+ * ------------------------------------------------------------------------- */
+
+/* out[k,z,x,y] = sum_{dz,dx,dy} mask[dz,dx,dy] * mu[k, z+dz-Mz/2, x+dx, y+dy-My/2],
+ * taps outside the volume skipped.  Replaces cal_atten_cuda (FL/cuda_lib/atten_cuda.cpp:12-28;
+ * kernel atten_cuda_kernel.cu:10-57).  mu, out (K,Z,X,Y); mask (Mz,Mx,My).  Any Y
+ * (the reference needs Y <= 1024). */
+FL_API int fl_cal_atten(const float *mu, const float *mask, float *out, int64_t K, int64_t Z,
+                        int64_t X, int64_t Y, int64_t Mz, int64_t Mx, int64_t My, void *stream);
+
+/* In-plane rotation of N planes (H,W): F.affine_grid([[c,-s,0],[s,c,0]]) then
+ * F.grid_sample (bilinear, zero padding, align_corners=False).  Replaces
+ * torch_rotate_image_nd with its defaults (FL/torch_image_util.py:337-438).
+ * cos_t, sin_t: the float32 matrix entries (the reference rounds math.cos(theta)). */
+FL_API int fl_rotate(const float *x, float *out, int64_t N, int64_t H, int64_t W, float cos_t,
+                     float sin_t, void *stream);
+
+/* Incident beam: out[e,b,H-1,c] = 1, out[e,b,r,c] = prod_{m=r+1}^{H-1} (1 - frac[e,b,m,c]*coef[e]).
+ * Replaces torch_xray_atten_3D (FL/torch_functions.py:83-106) with mu_x from
+ * torch_extract_3D_mu_xray (:8-52).  coef: device (K,) floats, or NULL when frac
+ * already holds mu_x.  Element strides (in floats) let frac/out be slices of
+ * larger arrays; B*H*W for dense (K,B,H,W). */
+FL_API int fl_incident_atten(const float *frac, const float *coef, float *out, int64_t K, int64_t B,
+                             int64_t H, int64_t W, int64_t frac_element_stride,
+                             int64_t out_element_stride, void *stream);
+
+/* mu at each fluorescence line, written in the mask frame:
+ *   mu_rot[e] = maskframe( sum_j frac[j] * coef[j*K + e] ),  frac (Kc,B,H,W), coef device (Kc,K),
+ *   mu_rot (K, B, W, H).  torch_extract_3D_mu_xrf (FL/torch_functions.py:55-81) followed by
+ * torch_rot90_4D (:142-145). */
+FL_API int fl_mu_xrf_rot(const float *frac, const float *coef, float *mu_rot, int64_t Kc,
+                         int64_t K, int64_t B, int64_t H, int64_t W, int side, void *stream);
+
+/* Direction-sampled detector cone in the mask frame (cone along +X):
+ *   T[k,z,x,y] = sum_d weight[d] * sum_{t>=1, x+t<X} bilin0(mu[k,.,x+t,.], z+t*pz[d]/q[d], y+t*py[d]/q[d])
+ * bilin0: bilinear with zero outside the volume.  Equals fl_cal_atten with the
+ * mask that splats these rays (every tap of such a mask lies on an integer voxel).
+ * Direction arrays are HOST memory; reduced denominators must be <= 32.
+ * No reference counterpart: the reference only has the dense mask form. */
+FL_API int fl_cone_atten(const float *mu_rot, float *T, int64_t K, int64_t Z, int64_t X, int64_t Y,
+                         const int32_t *pz, const int32_t *py, const int32_t *q,
+                         const float *weight, int64_t ndir, void *stream);
+
+/* out[e,b,r,c] = exp(-T_rot[e, b, maskframe(r,c)]) * (incident ? incident[e,b,r,c] : 1).
+ * The tail of torch_xrf_atten_3D (rot90 back and exp, FL/torch_functions.py:170-174)
+ * fused with the per-element product of cal_total_atten_at_angles (FL/core.py:99).
+ * T_rot (K,B,W,H) dense; incident/out element strides in floats. */
+FL_API int fl_xrf_combine(const float *T_rot, const float *incident, float *out, int64_t K,
+                          int64_t B, int64_t H, int64_t W, int64_t inc_element_stride,
+                          int64_t out_element_stride, int side, void *stream);
 
 #ifdef __cplusplus
 }
