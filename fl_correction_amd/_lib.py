@@ -46,6 +46,14 @@ SIGNATURES = {
     "fl_mu_xrf_rot": (_INT, [_P, _P, _P, _I64, _I64, _I64, _I64, _I64, _INT, _P]),
     "fl_cone_atten": (_INT, [_P, _P, _I64, _I64, _I64, _I64, _P, _P, _P, _P, _I64, _P]),
     "fl_xrf_combine": (_INT, [_P, _P, _P, _I64, _I64, _I64, _I64, _I64, _I64, _INT, _P]),
+    "fl_cal_atten_adjoint": (_INT, [_P, _P, _P, _I64, _I64, _I64, _I64, _I64, _I64, _I64, _P]),
+    "fl_cone_atten_adjoint": (_INT, [_P, _P, _I64, _I64, _I64, _I64, _P, _P, _P, _P, _I64, _P]),
+    "fl_backward_emission_accumulate": (_INT, [_P, _P, _P, _P, _P, _I64, _I64, _I64, _I64, _I64, _P]),
+    "fl_sample_grad": (_INT, [_P, _P, _P, _P, _P, _I64, _I64, _I64, _I64, _I64, _I64, _P]),
+    "fl_xrf_grad_prep": (_INT, [_P, _P, _P, _P, _P, _P, _I64, _I64, _I64, _I64, _I64, _INT, _P]),
+    "fl_incident_grad": (_INT, [_P, _P, _P, _P, _P, _I64, _I64, _I64, _I64, _P]),
+    "fl_mu_adjoint": (_INT, [_P, _P, _P, _P, _P, _I64, _I64, _I64, _I64, _I64, _INT, _P]),
+    "fl_rotate_adjoint": (_INT, [_P, _P, _I64, _I64, _I64, _F, _F, _INT, _P]),
 }
 
 

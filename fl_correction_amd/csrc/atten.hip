@@ -28,7 +28,9 @@ namespace fl {
 
 // ---------------------------------------------------------------------------
 // Dense mask correlation (atten_kernel, .cu:10-57):
-//   out[k,z,x,y] = sum_{dz,dx,dy} mask[dz,dx,dy] * mu[k, z+dz-Mz/2, x+dx, y+dy-My/2]
+//   out[k,z,x,y] = sum_{dz,dx,dy} mask[dz,dx,dy] * mu[k, z+dz-hz, x+dx-hx, y+dy-hy]
+// with (hz, hx, hy) = (Mz/2, 0, My/2) for the reference op; the transpose is the
+// same correlation with the mask flipped and (Mz-1-hz, Mx-1, My-1-hy).
 // One thread computes kXB outputs along x for one (k, z, y); lanes run along y,
 // so every mu load is coalesced and each loaded value feeds kXB multiply-adds.
 // Mask values are wave-uniform (scalar loads); zero taps are skipped.
@@ -37,7 +39,7 @@ constexpr int kXB = 8;
 
 __global__ __launch_bounds__(256) void mask_corr_kernel(const float *__restrict__ mu, const float *__restrict__ mask,
                                                         float *__restrict__ out, int K, int Z, int X, int Y, int Mz,
-                                                        int Mx, int My, int nxb)
+                                                        int Mx, int My, int hz, int hx, int hy, int flip, int nxb)
 {
     const int y = blockIdx.x * 64 + (threadIdx.x & 63);
     const int xb = blockIdx.y * 4 + (threadIdx.x >> 6);
@@ -45,7 +47,6 @@ __global__ __launch_bounds__(256) void mask_corr_kernel(const float *__restrict_
     const int k = kz / Z, z = kz % Z;
     if (xb >= nxb) return;
     const int x0 = xb * kXB;
-    const int hz = Mz / 2, hy = My / 2;
     const int64_t XY = (int64_t)X * Y;
     const float *muk = mu + (int64_t)k * Z * XY;
     float acc[kXB];
@@ -59,15 +60,19 @@ __global__ __launch_bounds__(256) void mask_corr_kernel(const float *__restrict_
         for (int dy = 0; dy < My; ++dy) {
             const int yy = y + dy - hy;
             const bool yok = yy >= 0 && yy < Y;
-            const float *mcol = mask + (int64_t)dz * Mx * My + dy;  // mask[dz, :, dy], stride My
-            const int mlast = min(kXB + Mx - 1, X - x0);
-            for (int m = 0; m < mlast; ++m) {
-                const float v = yok ? plane[(int64_t)(x0 + m) * Y + yy] : 0.0f;
+            // mask[dz, :, dy] with stride My, or the flipped mask M[Mz-1-dz, Mx-1-dx, My-1-dy]
+            const float *mcol = flip ? mask + ((int64_t)(Mz - 1 - dz) * Mx + (Mx - 1)) * My + (My - 1 - dy)
+                                     : mask + (int64_t)dz * Mx * My + dy;
+            const int64_t mstride = flip ? -(int64_t)My : (int64_t)My;
+            // input rows x0 - hx + m feed outputs x0 + j through tap dx = m - j
+            const int mfirst = max(0, hx - x0), mlast = min(kXB + Mx - 1, X - x0 + hx);
+            for (int m = mfirst; m < mlast; ++m) {
+                const float v = yok ? plane[(int64_t)(x0 - hx + m) * Y + yy] : 0.0f;
 #pragma unroll
                 for (int j = 0; j < kXB; ++j) {
                     const int dx = m - j;
                     if (dx >= 0 && dx < Mx) {
-                        const float mv = mcol[(int64_t)dx * My];
+                        const float mv = mcol[dx * mstride];
                         if (mv != 0.0f) acc[j] = fmaf(mv, v, acc[j]);
                     }
                 }
@@ -91,21 +96,15 @@ __global__ __launch_bounds__(256) void rotate_kernel(const float *__restrict__ x
     const int j = blockIdx.x * 64 + (threadIdx.x & 63);
     const int i = blockIdx.y * 4 + (threadIdx.x >> 6);
     if (j >= W || i >= H) return;
-    for (int64_t n = blockIdx.z; n < N; n += gridDim.z) {
-    const float xn = (float)(2 * j + 1) / (float)W - 1.0f;
-    const float yn = (float)(2 * i + 1) / (float)H - 1.0f;
-    const float gx = c * xn - s * yn;
-    const float gy = s * xn + c * yn;
-    const float ix = ((gx + 1.0f) * W - 1.0f) * 0.5f;
-    const float iy = ((gy + 1.0f) * H - 1.0f) * 0.5f;
-    const float fx0 = floorf(ix), fy0 = floorf(iy);
-    const int x0 = (int)fx0, y0 = (int)fy0;
-    const float e = ix - fx0, w_ = 1.0f - e;
-    const float d = iy - fy0, n_ = 1.0f - d;
-    const float *img = x + n * H * W;
-    float v = 0.0f;
+    int y0, x0;
+    float d, e;
+    rot_sample(i, j, H, W, c, s, y0, x0, d, e);
+    const float w_ = 1.0f - e, n_ = 1.0f - d;
     const bool x0ok = x0 >= 0 && x0 < W, x1ok = x0 + 1 >= 0 && x0 + 1 < W;
     const bool y0ok = y0 >= 0 && y0 < H, y1ok = y0 + 1 >= 0 && y0 + 1 < H;
+    for (int64_t n = blockIdx.z; n < N; n += gridDim.z) {
+    const float *img = x + n * H * W;
+    float v = 0.0f;
     if (y0ok && x0ok) v += img[(int64_t)y0 * W + x0] * (w_ * n_);
     if (y0ok && x1ok) v += img[(int64_t)y0 * W + x0 + 1] * (e * n_);
     if (y1ok && x0ok) v += img[(int64_t)(y0 + 1) * W + x0] * (w_ * d);
@@ -203,6 +202,8 @@ __global__ __launch_bounds__(256) void mu_rot_kernel(const float *__restrict__ f
 // its far end, so the recursion is exact (no re-interpolation of J).  The q-step
 // stencil (<= 4q taps with exact rational weights) comes from the host.
 //   T[v] (+)= weight * J(v)
+// reverse = 1 runs the transpose (the adjoint used by the gradient): taps with
+// negated offsets, chains walked from the near end, J(v) = S(v) + J(v - D).
 // ---------------------------------------------------------------------------
 constexpr int kMaxTaps = 128;
 
@@ -211,6 +212,7 @@ struct ConeTaps {
     int pz, py, q;
     float weight;
     int accumulate;
+    int reverse;
     int zlo, nz, ylo, ny;  // chain start ranges (mask frame)
     int8_t t[kMaxTaps];
     int8_t dz[kMaxTaps];
@@ -232,7 +234,8 @@ __global__ __launch_bounds__(256) void cone_kernel(const float *__restrict__ mu,
     float *Tk = T + (int64_t)k * Z * XY;
     const int M = (X - x0 + p.q - 1) / p.q;  // chain points with x < X
     float J = 0.0f;
-    for (int m = M - 1; m >= 0; --m) {
+    for (int it = 0; it < M; ++it) {
+        const int m = p.reverse ? it : M - 1 - it;
         const int x = x0 + m * p.q;
         const int z = z0 + m * p.pz;
         const int y = y0 + m * p.py;
@@ -249,7 +252,7 @@ __global__ __launch_bounds__(256) void cone_kernel(const float *__restrict__ mu,
             const int xx = x + p.t[i];
             const int zz = z + p.dz[i];
             const int yy = y + p.dy[i];
-            if (xx < X && zz >= 0 && zz < Z && yy >= 0 && yy < Y)
+            if (xx >= 0 && xx < X && zz >= 0 && zz < Z && yy >= 0 && yy < Y)
                 S = fmaf(p.w[i], muk[(int64_t)zz * XY + (int64_t)xx * Y + yy], S);
         }
         J += S;
@@ -300,13 +303,13 @@ static int64_t floordiv(int64_t a, int64_t b)
 }
 
 int launch_mask_corr(const float *mu, const float *mask, float *out, int64_t K, int64_t Z, int64_t X, int64_t Y,
-                     int64_t Mz, int64_t Mx, int64_t My, hipStream_t s)
+                     int64_t Mz, int64_t Mx, int64_t My, int64_t hz, int64_t hx, int64_t hy, int flip, hipStream_t s)
 {
     const int nxb = (int)((X + kXB - 1) / kXB);
     dim3 grid((unsigned)((Y + 63) / 64), (unsigned)((nxb + 3) / 4), (unsigned)(K * Z));
     FL_REQUIRE(K * Z <= 65535, "mask correlation: K*Z must be <= 65535");
     hipLaunchKernelGGL(mask_corr_kernel, grid, dim3(256), 0, s, mu, mask, out, (int)K, (int)Z, (int)X, (int)Y,
-                       (int)Mz, (int)Mx, (int)My, nxb);
+                       (int)Mz, (int)Mx, (int)My, (int)hz, (int)hx, (int)hy, flip, nxb);
     FL_LAUNCH_CHECK();
     return FL_OK;
 }
@@ -352,7 +355,7 @@ int launch_combine(const float *T_rot, const float *incident, float *out, int64_
 
 // Builds the q-step stencil of direction (pz, py, q) and launches one chain walk.
 int launch_cone_direction(const float *mu, float *T, int64_t K, int64_t Z, int64_t X, int64_t Y, int pz, int py,
-                          int q, float weight, int accumulate, hipStream_t s)
+                          int q, float weight, int accumulate, int reverse, hipStream_t s)
 {
     FL_REQUIRE(q >= 1, "cone direction: q must be >= 1");
     // reduce the rational slopes so the chain step is as short as possible
@@ -370,6 +373,8 @@ int launch_cone_direction(const float *mu, float *T, int64_t K, int64_t Z, int64
     p.q = q;
     p.weight = weight;
     p.accumulate = accumulate;
+    p.reverse = reverse;
+    const int sg = reverse ? -1 : 1;
     int n = 0;
     for (int t = 1; t <= q; ++t) {
         const int64_t nz = (int64_t)t * pz, ny = (int64_t)t * py;
@@ -381,9 +386,9 @@ int launch_cone_direction(const float *mu, float *T, int64_t K, int64_t Z, int64
                 if (wz == 0.0 || wy == 0.0) continue;
                 FL_REQUIRE(n < kMaxTaps && std::abs(z0 + a) < 127 && std::abs(y0 + b) < 127,
                            "cone direction: stencil too large");
-                p.t[n] = (int8_t)t;
-                p.dz[n] = (int8_t)(z0 + a);
-                p.dy[n] = (int8_t)(y0 + b);
+                p.t[n] = (int8_t)(sg * t);
+                p.dz[n] = (int8_t)(sg * (z0 + a));
+                p.dy[n] = (int8_t)(sg * (y0 + b));
                 p.w[n] = (float)(wz * wy);
                 ++n;
             }

@@ -19,7 +19,18 @@ int launch_backward(const float *atten, const float *ratio, const float *em, con
 int launch_huber(const float *C, float *grad, int64_t n_slices, int64_t H, int64_t W, float delta,
                  hipStream_t stream);
 int launch_mask_corr(const float *mu, const float *mask, float *out, int64_t K, int64_t Z, int64_t X, int64_t Y,
-                     int64_t Mz, int64_t Mx, int64_t My, hipStream_t s);
+                     int64_t Mz, int64_t Mx, int64_t My, int64_t hz, int64_t hx, int64_t hy, int flip, hipStream_t s);
+int launch_sample_grad(const float *C, const float *em, const float *theta, const float *G, float *out, int64_t A,
+                       int64_t B, int64_t R, int64_t H, int64_t W, int64_t out_astride, hipStream_t s);
+int launch_xrf_grad_prep(const float *T_rot, const float *Ax, const float *gA, float *dT_rot, float *gx,
+                         float *A_tot, int64_t A_tot_estride, int64_t K, int64_t B, int64_t H, int64_t W, int side,
+                         hipStream_t s);
+int launch_incident_grad(const float *F, const float *coef_x, const float *Ax, const float *gx, float *dmux,
+                         int64_t K, int64_t B, int64_t H, int64_t W, hipStream_t s);
+int launch_mu_adjoint(const float *dmu_rot, const float *dmux, const float *coef, const float *coef_x, float *dF,
+                      int64_t Kc, int64_t K, int64_t B, int64_t H, int64_t W, int side, hipStream_t s);
+int launch_rotate_adjoint(const float *dF, float *dC, int64_t N, int64_t H, int64_t W, float c, float s_,
+                          int accumulate, hipStream_t s);
 int launch_rotate(const float *x, float *out, int64_t N, int64_t H, int64_t W, float c, float s_, hipStream_t s);
 int launch_incident(const float *frac, const float *coef, float *out, int64_t frac_estride, int64_t out_estride,
                     int64_t K, int64_t B, int64_t H, int64_t W, hipStream_t s);
@@ -28,7 +39,7 @@ int launch_mu_rot(const float *frac, const float *coef, float *mu_rot, int64_t K
 int launch_combine(const float *T_rot, const float *incident, float *out, int64_t inc_estride, int64_t out_estride,
                    int64_t K, int64_t B, int64_t H, int64_t W, int side, hipStream_t s);
 int launch_cone_direction(const float *mu, float *T, int64_t K, int64_t Z, int64_t X, int64_t Y, int pz, int py,
-                          int q, float weight, int accumulate, hipStream_t s);
+                          int q, float weight, int accumulate, int reverse, hipStream_t s);
 
 static int check_dims(int64_t A, int64_t B, int64_t R, int64_t H, int64_t W)
 {
@@ -204,7 +215,20 @@ int fl_cal_atten(const float *mu, const float *mask, float *out, int64_t K, int6
     if (K == 0 || Z == 0 || X == 0 || Y == 0) return FL_OK;
     FL_REQUIRE(mu && mask && out, "null pointer argument");
     FL_REQUIRE(mu != out, "out must not alias mu");
-    return launch_mask_corr(mu, mask, out, K, Z, X, Y, Mz, Mx, My, (hipStream_t)stream);
+    return launch_mask_corr(mu, mask, out, K, Z, X, Y, Mz, Mx, My, Mz / 2, 0, My / 2, 0, (hipStream_t)stream);
+}
+
+int fl_cal_atten_adjoint(const float *g, const float *mask, float *out, int64_t K, int64_t Z, int64_t X, int64_t Y,
+                         int64_t Mz, int64_t Mx, int64_t My, void *stream)
+{
+    g_last_error.clear();
+    FL_REQUIRE(K >= 0 && Z >= 0 && X >= 0 && Y >= 0 && Mz >= 1 && Mx >= 1 && My >= 1, "shape: negative size");
+    FL_REQUIRE(X < (1 << 24) && Y < (1 << 24) && Z < (1 << 24), "shape: dimension too large");
+    if (K == 0 || Z == 0 || X == 0 || Y == 0) return FL_OK;
+    FL_REQUIRE(g && mask && out && g != out, "null or aliased pointer argument");
+    // transpose of the correlation: flipped mask, centres Mz-1-Mz/2, Mx-1, My-1-My/2
+    return launch_mask_corr(g, mask, out, K, Z, X, Y, Mz, Mx, My, Mz - 1 - Mz / 2, Mx - 1, My - 1 - My / 2, 1,
+                            (hipStream_t)stream);
 }
 
 int fl_rotate(const float *x, float *out, int64_t N, int64_t H, int64_t W, float cos_t, float sin_t, void *stream)
@@ -249,10 +273,95 @@ int fl_cone_atten(const float *mu_rot, float *T, int64_t K, int64_t Z, int64_t X
     FL_REQUIRE(mu_rot && T && pz && py && q && weight, "null pointer argument");
     FL_REQUIRE(mu_rot != T, "T must not alias mu");
     for (int64_t d = 0; d < ndir; ++d)
-        if (int e = launch_cone_direction(mu_rot, T, K, Z, X, Y, pz[d], py[d], q[d], weight[d], d > 0,
+        if (int e = launch_cone_direction(mu_rot, T, K, Z, X, Y, pz[d], py[d], q[d], weight[d], d > 0, 0,
                                           (hipStream_t)stream))
             return e;
     return FL_OK;
+}
+
+int fl_cone_atten_adjoint(const float *g_rot, float *dmu_rot, int64_t K, int64_t Z, int64_t X, int64_t Y,
+                          const int32_t *pz, const int32_t *py, const int32_t *q, const float *weight, int64_t ndir,
+                          void *stream)
+{
+    g_last_error.clear();
+    FL_REQUIRE(K >= 1 && Z >= 1 && X >= 1 && Y >= 1 && ndir >= 1, "shape: bad size");
+    FL_REQUIRE(Z < (1 << 20) && X < (1 << 20) && Y < (1 << 20) && K < 2048, "shape: too large");
+    FL_REQUIRE(g_rot && dmu_rot && pz && py && q && weight, "null pointer argument");
+    FL_REQUIRE(g_rot != dmu_rot, "output must not alias input");
+    for (int64_t d = 0; d < ndir; ++d)
+        if (int e = launch_cone_direction(g_rot, dmu_rot, K, Z, X, Y, pz[d], py[d], q[d], weight[d], d > 0, 1,
+                                          (hipStream_t)stream))
+            return e;
+    return FL_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Adjoint of the attenuated forward model
+// ---------------------------------------------------------------------------
+
+int fl_backward_emission_accumulate(const float *atten, const float *ratio, const float *em_cs,
+                                    const float *theta, float *out, int64_t A, int64_t B, int64_t R, int64_t H,
+                                    int64_t W, void *stream)
+{
+    g_last_error.clear();
+    if (int e = check_dims(A, B, R, H, W)) return e;
+    if (A == 0 || B == 0) return FL_OK;
+    FL_REQUIRE(atten && ratio && em_cs && theta && out, "null pointer argument");
+    return launch_backward(atten, ratio, em_cs, theta, out, A, B, R, H, W, 3, nullptr, nullptr, 0.f, 0.f,
+                           (hipStream_t)stream);
+}
+
+int fl_sample_grad(const float *C, const float *em_cs, const float *theta, const float *G, float *out, int64_t A,
+                   int64_t B, int64_t R, int64_t H, int64_t W, int64_t out_angle_stride, void *stream)
+{
+    g_last_error.clear();
+    if (int e = check_dims(A, B, R, H, W)) return e;
+    if (A == 0 || B == 0) return FL_OK;
+    FL_REQUIRE(C && em_cs && theta && G && out, "null pointer argument");
+    return launch_sample_grad(C, em_cs, theta, G, out, A, B, R, H, W, out_angle_stride, (hipStream_t)stream);
+}
+
+int fl_xrf_grad_prep(const float *T_rot, const float *Ax, const float *gA, float *dT_rot, float *gx, float *A_tot,
+                     int64_t A_tot_element_stride, int64_t K, int64_t B, int64_t H, int64_t W, int side,
+                     void *stream)
+{
+    g_last_error.clear();
+    FL_REQUIRE(K >= 1 && B >= 0 && H >= 1 && W >= 1 && (side == 0 || side == 1), "bad arguments");
+    if (B == 0) return FL_OK;
+    FL_REQUIRE(T_rot && Ax && gA && dT_rot && gx, "null pointer argument");
+    return launch_xrf_grad_prep(T_rot, Ax, gA, dT_rot, gx, A_tot, A_tot_element_stride, K, B, H, W, side,
+                                (hipStream_t)stream);
+}
+
+int fl_incident_grad(const float *F, const float *coef_x, const float *Ax, const float *gx, float *dmux, int64_t K,
+                     int64_t B, int64_t H, int64_t W, void *stream)
+{
+    g_last_error.clear();
+    FL_REQUIRE(K >= 1 && B >= 0 && H >= 1 && W >= 1 && K < 65536 && B < 65536, "bad arguments");
+    if (B == 0) return FL_OK;
+    FL_REQUIRE(F && coef_x && Ax && gx && dmux, "null pointer argument");
+    return launch_incident_grad(F, coef_x, Ax, gx, dmux, K, B, H, W, (hipStream_t)stream);
+}
+
+int fl_mu_adjoint(const float *dmu_rot, const float *dmux, const float *coef, const float *coef_x, float *dF,
+                  int64_t Kc, int64_t K, int64_t B, int64_t H, int64_t W, int side, void *stream)
+{
+    g_last_error.clear();
+    FL_REQUIRE(Kc >= 1 && K >= 1 && B >= 0 && B < 65536 && H >= 1 && W >= 1 && (side == 0 || side == 1),
+               "bad arguments");
+    if (B == 0) return FL_OK;
+    FL_REQUIRE(dmu_rot && coef && dF && (!dmux || coef_x), "null pointer argument");
+    return launch_mu_adjoint(dmu_rot, dmux, coef, coef_x, dF, Kc, K, B, H, W, side, (hipStream_t)stream);
+}
+
+int fl_rotate_adjoint(const float *dF, float *dC, int64_t N, int64_t H, int64_t W, float cos_t, float sin_t,
+                      int accumulate, void *stream)
+{
+    g_last_error.clear();
+    FL_REQUIRE(N >= 0 && H >= 1 && W >= 1 && H < (1 << 20) && W < (1 << 20), "shape: bad size");
+    if (N == 0) return FL_OK;
+    FL_REQUIRE(dF && dC && dF != dC, "null or aliased pointer argument");
+    return launch_rotate_adjoint(dF, dC, N, H, W, cos_t, sin_t, accumulate, (hipStream_t)stream);
 }
 
 int fl_xrf_combine(const float *T_rot, const float *incident, float *out, int64_t K, int64_t B, int64_t H,

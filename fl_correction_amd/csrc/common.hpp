@@ -37,6 +37,26 @@ __device__ __forceinline__ float clampf(float v, float lo, float hi)
 // every index stays inside the arrays.
 __device__ __forceinline__ float finite_or_zero(float v) { return isfinite(v) ? v : 0.0f; }
 
+// Sample point of output pixel (i, j) for torch's affine_grid([[c,-s,0],[s,c,0]])
+// + grid_sample(align_corners=False) (FL/torch_image_util.py:401-428): the
+// input position (iy, ix) split into its integer corner (y0, x0) and fractions.
+// Shared by the rotation and its transpose so both use identical arithmetic.
+__device__ __forceinline__ void rot_sample(int i, int j, int H, int W, float c, float s, int &y0, int &x0,
+                                           float &fy, float &fx)
+{
+    const float xn = (float)(2 * j + 1) / (float)W - 1.0f;
+    const float yn = (float)(2 * i + 1) / (float)H - 1.0f;
+    const float gx = c * xn - s * yn;
+    const float gy = s * xn + c * yn;
+    const float ix = ((gx + 1.0f) * W - 1.0f) * 0.5f;
+    const float iy = ((gy + 1.0f) * H - 1.0f) * 0.5f;
+    const float fx0 = floorf(ix), fy0 = floorf(iy);
+    x0 = (int)fx0;
+    y0 = (int)fy0;
+    fx = ix - fx0;
+    fy = iy - fy0;
+}
+
 __device__ __forceinline__ int uniform_int(int v) { return __builtin_amdgcn_readfirstlane(v); }
 
 __device__ __forceinline__ float uniform_float(float v)

@@ -250,7 +250,7 @@ constexpr int kBAcc = (kBI + 2) * kBPitch;
 constexpr int kBRefChunk = 2;
 constexpr int kBUnroll = 4;           // 8x8 blocks loaded together
 
-enum BwdMode { kBwdWrite = 0, kBwdSens = 1, kBwdUpdate = 2 };
+enum BwdMode { kBwdWrite = 0, kBwdSens = 1, kBwdUpdate = 2, kBwdAccumulate = 3 };
 
 struct BwdArgs {
     const float *atten;
@@ -424,6 +424,8 @@ __global__ __launch_bounds__(256) void bwd_tile_kernel(BwdArgs p)
             const int64_t o = sb + (int64_t)i * p.W + j;
             if (p.mode == kBwdWrite) {
                 p.out[o] = back;
+            } else if (p.mode == kBwdAccumulate) {
+                p.out[o] += back;
             } else if (p.mode == kBwdSens) {
                 p.out[o] = back < 1e-6f ? 1e-6f : back;  // sens.clamp_min_(1e-6)
             } else {

@@ -9,3 +9,5 @@ from .atten_cuda import (
     mlem_cuda,
     mlem_cuda_batch,
 )
+
+from .autograd import forward_emission_autograd, forward_emission_batch_autograd

@@ -1,0 +1,228 @@
+"""Differentiable attenuated XRF forward model on the GPU.
+
+For concentrations C (K, B, H, W) -- one map per element, each element carrying
+its own emission line -- and angles theta_a:
+
+    F_j   = Rot_a(C_j)                                (torch_rotate_image_nd, FL/core.py:43)
+    A_x,e = prod_{m>r} (1 - coef_x[a,e] F_e[m])       (torch_xray_atten_3D, FL/torch_functions.py:83-106)
+    A_f,e = exp(-Cone(sum_j coef[a,j,e] F_j))         (torch_xrf_atten_3D, FL/torch_functions.py:110-176)
+    A_e   = A_f,e * A_x,e                             (cal_total_atten_at_angles, FL/core.py:84-101)
+    Pf_e  = em[a,e] * sum_h A_e * S_a(C_e)            (forward_emission_batch, FL/cuda_lib/atten_cuda.cpp:142)
+
+``forward`` returns Pf (K, A, B, W).  ``vjp`` returns dL/dC for a cotangent
+G = dL/dPf, including the dependence of the attenuation on C that the
+reference's autograd wrapper (FL/cuda_lib/autograd.py:40-74) leaves out.
+Attenuation exponents are kept for the backward pass when they fit the memory
+budget and recomputed per angle otherwise.
+"""
+from __future__ import annotations
+
+import numpy as np
+import torch
+
+from . import _lib
+from .atten_util import coef_xray, coef_xrf, extract_single_value_param
+from .core import angle_entries
+from .detector_mask import Cone
+
+
+def _f32(x, dev):
+    return torch.as_tensor(np.ascontiguousarray(x, dtype=np.float32), device=dev)
+
+
+class XRFModel:
+    """Attenuated emission model for a fixed geometry.
+
+    param:      the parameter dictionary (atten_util.make_param / load_param layout)
+    angle_list: angles in degrees
+    detector:   a dense mask (Mz, Mx, My) or a Cone
+    em:         (A, K) emission factors; default em_cs[elem] * pix per element
+    store:      'auto' | True | False -- keep the fluorescence exponents between forward and vjp
+    """
+
+    def __init__(self, param, angle_list, detector, em=None, position_det="r", store="auto",
+                 memory_fraction=0.4, device="cuda"):
+        self.device = torch.device(device)
+        self.angles = np.asarray(angle_list, np.float64).reshape(-1)
+        A = len(self.angles)
+        elem = param["elem_type"]
+        K = len(elem)
+        self.A, self.K = A, K
+        cx, cf = [], []
+        for a in range(A):
+            p = extract_single_value_param(param, a)
+            cx.append(coef_xray(p))
+            cf.append(coef_xrf(p))
+        self.coef_x = _f32(np.stack(cx), self.device)       # (A, K)
+        self.coef_xrf = _f32(np.stack(cf), self.device)     # (A, Kc, K)
+        if em is None:
+            em = np.zeros((A, K))
+            for e, el in enumerate(elem):
+                v = np.atleast_1d(np.asarray(param["em_cs"][el], np.float64))
+                em[:, e] = (v if v.size == A else np.full(A, v[0])) * param["pix"]
+        em = np.asarray(em, np.float32).reshape(A, K)
+        self.em_T = _f32(em.T, self.device)                # (K, A): per-line, contiguous
+        self.theta = _f32(self.angles / 180.0 * np.pi, self.device)
+        self.trig = [angle_entries(a) for a in self.angles]
+        self.side = {"r": 0, "l": 1}[position_det]
+        self.detector = detector
+        if isinstance(detector, Cone):
+            self._dirs = [np.ascontiguousarray(v) for v in (detector.pz, detector.py, detector.qd, detector.weight)]
+        else:
+            self.mask = _f32(detector, self.device)
+        self.store = store
+        self.memory_fraction = memory_fraction
+        self._T_store = None
+
+    # ------------------------------------------------------------------
+    def _st(self):
+        return _lib.stream_of(self.device)
+
+    def _dir_ptrs(self):
+        return [d.ctypes.data_as(_lib.ctypes.c_void_p) for d in self._dirs]
+
+    def _exponent(self, mu_rot, T_rot, shape):
+        K, B, H, W = shape
+        if isinstance(self.detector, Cone):
+            _lib.call("fl_cone_atten", _lib.ptr(mu_rot), _lib.ptr(T_rot), K, B, W, H, *self._dir_ptrs(),
+                      self.detector.ndir, self._st())
+        else:
+            Mz, Mx, My = self.mask.shape
+            _lib.call("fl_cal_atten", _lib.ptr(mu_rot), _lib.ptr(self.mask), _lib.ptr(T_rot), K, B, W, H, Mz, Mx, My,
+                      self._st())
+
+    def _exponent_adjoint(self, g_rot, out, shape):
+        K, B, H, W = shape
+        if isinstance(self.detector, Cone):
+            _lib.call("fl_cone_atten_adjoint", _lib.ptr(g_rot), _lib.ptr(out), K, B, W, H, *self._dir_ptrs(),
+                      self.detector.ndir, self._st())
+        else:
+            Mz, Mx, My = self.mask.shape
+            _lib.call("fl_cal_atten_adjoint", _lib.ptr(g_rot), _lib.ptr(self.mask), _lib.ptr(out), K, B, W, H, Mz, Mx,
+                      My, self._st())
+
+    def _angle_maps(self, C, a, F, Ax, mu_rot, T_rot, recompute_T=True):
+        """F = Rot(C), Ax = incident, T_rot = exponent (mask frame) for angle index a."""
+        K, B, H, W = C.shape
+        c, s = self.trig[a]
+        st = self._st()
+        _lib.call("fl_rotate", _lib.ptr(C), _lib.ptr(F), K * B, H, W, c, s, st)
+        _lib.call("fl_incident_atten", _lib.ptr(F), _lib.ptr(self.coef_x[a]), _lib.ptr(Ax), K, B, H, W, B * H * W,
+                  B * H * W, st)
+        if recompute_T:
+            _lib.call("fl_mu_xrf_rot", _lib.ptr(F), _lib.ptr(self.coef_xrf[a]), _lib.ptr(mu_rot), K, K, B, H, W,
+                      self.side, st)
+            self._exponent(mu_rot, T_rot, (K, B, H, W))
+
+    def _use_store(self, C):
+        if self.store == "auto":
+            need = 4 * self.K * self.A * C[0].numel()
+            free, _ = torch.cuda.mem_get_info(self.device)
+            return need <= self.memory_fraction * free
+        return bool(self.store)
+
+    # ------------------------------------------------------------------
+    def attenuation(self, C, out=None):
+        """Total attenuation (K, A, B, H, W) for concentrations C (K, B, H, W)."""
+        C = C.to(self.device, torch.float32).contiguous()
+        K, B, H, W = C.shape
+        if out is None:
+            out = torch.empty((K, self.A, B, H, W), dtype=torch.float32, device=self.device)
+        F, Ax = torch.empty_like(C), torch.empty_like(C)
+        mu_rot = torch.empty((K, B, W, H), dtype=torch.float32, device=self.device)
+        T_rot = torch.empty_like(mu_rot)
+        with torch.cuda.device(self.device):
+            for a in range(self.A):
+                self._angle_maps(C, a, F, Ax, mu_rot, T_rot)
+                _lib.call("fl_xrf_combine", _lib.ptr(T_rot), _lib.ptr(Ax), _lib.ptr(out[0, a]), K, B, H, W, B * H * W,
+                          out.stride(0), self.side, self._st())
+        return out
+
+    def forward(self, C, angle_chunk=16):
+        """Pf (K, A, B, W).  Keeps the fluorescence exponents for vjp when storing."""
+        C = C.to(self.device, torch.float32).contiguous()
+        K, B, H, W = C.shape
+        if K != self.K:
+            raise RuntimeError(f"C has {K} element maps, the model {self.K}")
+        store = self._use_store(C)
+        self._T_store = torch.empty((self.A, K, B, W, H), dtype=torch.float32, device=self.device) if store else None
+        Pf = torch.empty((K, self.A, B, W), dtype=torch.float32, device=self.device)
+        nA = min(angle_chunk, self.A)
+        Atot = torch.empty((K, nA, B, H, W), dtype=torch.float32, device=self.device)
+        F, Ax = torch.empty_like(C), torch.empty_like(C)
+        mu_rot = torch.empty((K, B, W, H), dtype=torch.float32, device=self.device)
+        T_tmp = torch.empty_like(mu_rot)
+        st = self._st()
+        with torch.cuda.device(self.device):
+            for a0 in range(0, self.A, nA):
+                a1 = min(a0 + nA, self.A)
+                for a in range(a0, a1):
+                    T_rot = self._T_store[a] if store else T_tmp
+                    self._angle_maps(C, a, F, Ax, mu_rot, T_rot)
+                    _lib.call("fl_xrf_combine", _lib.ptr(T_rot), _lib.ptr(Ax), _lib.ptr(Atot[0, a - a0]), K, B, H, W,
+                              B * H * W, Atot.stride(0), self.side, st)
+                for e in range(K):
+                    _lib.call("fl_forward_emission_batch", _lib.ptr(Atot[e]), _lib.ptr(C[e]), _lib.ptr(self.em_T[e, a0:]),
+                              _lib.ptr(self.theta[a0:]), _lib.ptr(Pf[e, a0]), a1 - a0, B, 1, H, W, st)
+        return Pf
+
+    def vjp(self, C, G, angle_chunk=16):
+        """dL/dC (K, B, H, W) for G = dL/dPf (K, A, B, W)."""
+        C = C.to(self.device, torch.float32).contiguous()
+        G = G.to(self.device, torch.float32).contiguous()
+        K, B, H, W = C.shape
+        if tuple(G.shape) != (K, self.A, B, W):
+            raise RuntimeError(f"G must be {(K, self.A, B, W)}")
+        stored = self._T_store is not None
+        dC = torch.zeros_like(C)
+        nA = min(angle_chunk, self.A)
+        Atot = torch.empty((K, nA, B, H, W), dtype=torch.float32, device=self.device)
+        F, Ax, gA, gx, dmux, dF = (torch.empty_like(C) for _ in range(6))
+        mu_rot = torch.empty((K, B, W, H), dtype=torch.float32, device=self.device)
+        T_tmp, dT_rot, dmu_rot = torch.empty_like(mu_rot), torch.empty_like(mu_rot), torch.empty_like(mu_rot)
+        st = self._st()
+        with torch.cuda.device(self.device):
+            for a0 in range(0, self.A, nA):
+                a1 = min(a0 + nA, self.A)
+                for a in range(a0, a1):
+                    T_rot = self._T_store[a] if stored else T_tmp
+                    self._angle_maps(C, a, F, Ax, mu_rot, T_rot, recompute_T=not stored)
+                    for e in range(K):
+                        _lib.call("fl_sample_grad", _lib.ptr(C[e]), _lib.ptr(self.em_T[e, a:]), _lib.ptr(self.theta[a:]),
+                                  _lib.ptr(G[e, a]), _lib.ptr(gA[e]), 1, B, 1, H, W, 0, st)
+                    _lib.call("fl_xrf_grad_prep", _lib.ptr(T_rot), _lib.ptr(Ax), _lib.ptr(gA), _lib.ptr(dT_rot),
+                              _lib.ptr(gx), _lib.ptr(Atot[0, a - a0]), Atot.stride(0), K, B, H, W, self.side, st)
+                    self._exponent_adjoint(dT_rot, dmu_rot, (K, B, H, W))
+                    _lib.call("fl_incident_grad", _lib.ptr(F), _lib.ptr(self.coef_x[a]), _lib.ptr(Ax), _lib.ptr(gx),
+                              _lib.ptr(dmux), K, B, H, W, st)
+                    _lib.call("fl_mu_adjoint", _lib.ptr(dmu_rot), _lib.ptr(dmux), _lib.ptr(self.coef_xrf[a]),
+                              _lib.ptr(self.coef_x[a]), _lib.ptr(dF), K, K, B, H, W, self.side, st)
+                    c, s = self.trig[a]
+                    _lib.call("fl_rotate_adjoint", _lib.ptr(dF), _lib.ptr(dC), K * B, H, W, c, s, 1, st)
+                for e in range(K):
+                    _lib.call("fl_backward_emission_accumulate", _lib.ptr(Atot[e]), _lib.ptr(G[e, a0]),
+                              _lib.ptr(self.em_T[e, a0:]), _lib.ptr(self.theta[a0:]), _lib.ptr(dC[e]), a1 - a0, B, 1,
+                              H, W, st)
+        return dC
+
+    def release(self):
+        self._T_store = None
+
+
+class XRFModelFunction(torch.autograd.Function):
+    """Pf = model(C) with the hand-written adjoint as its backward."""
+
+    @staticmethod
+    def forward(ctx, C, model):
+        ctx.model = model
+        ctx.save_for_backward(C)
+        return model.forward(C.detach())
+
+    @staticmethod
+    def backward(ctx, G):
+        (C,) = ctx.saved_tensors
+        return ctx.model.vjp(C.detach(), G), None
+
+
+def xrf_forward_autograd(C, model):
+    return XRFModelFunction.apply(C, model)

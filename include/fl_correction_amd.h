@@ -173,6 +173,62 @@ FL_API int fl_xrf_combine(const float *T_rot, const float *incident, float *out,
                           int64_t B, int64_t H, int64_t W, int64_t inc_element_stride,
                           int64_t out_element_stride, int side, void *stream);
 
+/* Transpose of fl_cal_atten: out = M^T (*) g, i.e. the correlation with the
+ * flipped mask.  No reference counterpart (the reference differentiates nothing
+ * through the fluorescence path); used by the adjoint of the attenuated model. */
+FL_API int fl_cal_atten_adjoint(const float *g, const float *mask, float *out, int64_t K,
+                                int64_t Z, int64_t X, int64_t Y, int64_t Mz, int64_t Mx,
+                                int64_t My, void *stream);
+
+/* Transpose of fl_cone_atten (the march run backwards with negated offsets).
+ * No reference counterpart. */
+FL_API int fl_cone_atten_adjoint(const float *g_rot, float *dmu_rot, int64_t K, int64_t Z,
+                                 int64_t X, int64_t Y, const int32_t *pz, const int32_t *py,
+                                 const int32_t *q, const float *weight, int64_t ndir,
+                                 void *stream);
+
+/* ---------------------------------------------------------------------------
+ * Adjoint of the attenuated emission model.  The reference differentiates the
+ * projector with respect to C only, holding the attenuation fixed
+ * (ForwardEmissionBatchFunction, FL/cuda_lib/autograd.py:40-74, backward =
+ * backward_emission_batch); these add d/d(atten) and the chain through the
+ * attenuation's dependence on the concentrations.
+ * ------------------------------------------------------------------------- */
+
+/* out += B^T(...): fl_backward_emission_batch accumulating into `out`. */
+FL_API int fl_backward_emission_accumulate(const float *atten, const float *ratio,
+                                           const float *em_cs, const float *theta, float *out,
+                                           int64_t A, int64_t B, int64_t R, int64_t H, int64_t W,
+                                           void *stream);
+
+/* d Pf / d atten contracted with G:  out[a,b,h,w] = G[a,b,w] * sum_r em_cs[a,r] * S_a(C[r,b])(h,w),
+ * S_a the projector's clamp-bilinear sample.  C (R,B,H,W), G (A,B,W); out planes for
+ * angle a start at a*out_angle_stride floats. */
+FL_API int fl_sample_grad(const float *C, const float *em_cs, const float *theta, const float *G,
+                          float *out, int64_t A, int64_t B, int64_t R, int64_t H, int64_t W,
+                          int64_t out_angle_stride, void *stream);
+
+/* Per line e: Axrf = exp(-T), dT_rot = -gA*Axrf*Ax (mask frame), gx = gA*Axrf,
+ * and (if A_tot != NULL) A_tot = Axrf*Ax.  T_rot, dT_rot (K,B,W,H); Ax, gA, gx (K,B,H,W). */
+FL_API int fl_xrf_grad_prep(const float *T_rot, const float *Ax, const float *gA, float *dT_rot,
+                            float *gx, float *A_tot, int64_t A_tot_element_stride, int64_t K,
+                            int64_t B, int64_t H, int64_t W, int side, void *stream);
+
+/* Adjoint of the incident product: dmux[m] = -Ax[m]*U[m], U[0] = 0,
+ * U[m+1] = U[m]*(1 - coef_x[e]*F[m]) + gx[m]  (division-free).  F, Ax, gx, dmux (K,B,H,W). */
+FL_API int fl_incident_grad(const float *F, const float *coef_x, const float *Ax, const float *gx,
+                            float *dmux, int64_t K, int64_t B, int64_t H, int64_t W, void *stream);
+
+/* dF[j] = sum_e coef[j*K+e] * dmu_rot[e] (mask frame to original) + coef_x[j]*dmux[j];
+ * dmux may be NULL.  dmu_rot (K,B,W,H); dmux (K,B,H,W); dF (Kc,B,H,W); K <= 8. */
+FL_API int fl_mu_adjoint(const float *dmu_rot, const float *dmux, const float *coef,
+                         const float *coef_x, float *dF, int64_t Kc, int64_t K, int64_t B,
+                         int64_t H, int64_t W, int side, void *stream);
+
+/* Transpose of fl_rotate: dC (+)= Rot^T(dF) over N planes, by gather (no atomics). */
+FL_API int fl_rotate_adjoint(const float *dF, float *dC, int64_t N, int64_t H, int64_t W,
+                             float cos_t, float sin_t, int accumulate, void *stream);
+
 #ifdef __cplusplus
 }
 #endif
