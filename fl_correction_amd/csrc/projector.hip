@@ -18,14 +18,16 @@
 // a fixed order at the end and written out directly, or turned into the MLEM
 // ratio I / max(Pf, 1e-6).  No atomics, no partial-sum buffer: deterministic.
 //
-// Back-projection: one workgroup owns a 32x64 image tile of one slice; each of
-// its 4 waves keeps a private LDS accumulator (tile plus a one-voxel halo).  For
-// every angle the waves split the rows of 8x8 detector-point blocks that cover
-// the tile's preimage (per block row, the column interval comes from the
-// rotated region's edges), load four blocks at a time (atten and ratio), and
-// splat the bilinear weights of the points that land in the tile into their
-// own accumulator.  No global atomics; the four accumulators are summed in a
-// fixed order at the end, which is also where the MLEM update is applied.
+// Back-projection: one workgroup owns a 31x63 image tile of one slice (the
+// clamped samples that touch it fill a 32x64 region); each of its 4 waves keeps
+// a private LDS accumulator (tile plus a one-voxel halo).  For every angle the
+// waves take turns over the detector rows of the tile's preimage: each row's
+// column interval comes from the rotated region's edges, and the row's points
+// are one coalesced segment of atten (lane = column; two rows per instruction
+// when rows are at most 32 long).  Four segments are loaded together, then the
+// bilinear weights of the points that land in the tile are splatted into the
+// wave's own accumulator.  No global atomics; the four accumulators are summed in
+// a fixed order at the end, which is also where the MLEM update is applied.
 #include "common.hpp"
 
 #include <algorithm>
@@ -94,7 +96,7 @@ __device__ __forceinline__ Box fwd_footprint(float c, float s, int h0, int h1, i
     return b;
 }
 
-__global__ __launch_bounds__(256) void fwd_strip_kernel(FwdArgs p)
+__global__ __launch_bounds__(256, 2) void fwd_strip_kernel(FwdArgs p)
 {
     extern __shared__ float lds[];
     __shared__ float trig[2 * kFG];
@@ -139,6 +141,17 @@ __global__ __launch_bounds__(256) void fwd_strip_kernel(FwdArgs p)
     for (int h0 = 0; h0 < p.H; h0 += kFH) {
         const int h1 = min(h0 + kFH, p.H);
         const int hr0 = h0 + wave * kFRows;
+        // This chunk's atten rows for all angles of the group, issued before the
+        // footprint staging so that their latency overlaps it.
+        float av[kFG][kFRows];
+#pragma unroll
+        for (int k = 0; k < kFG; ++k) {
+            const float *ap = p.atten + ((int64_t)(a0 + min(k, na - 1)) * p.B + b) * HW + (int64_t)hr0 * p.W + w;
+#pragma unroll
+            for (int i = 0; i < kFRows; ++i)
+                av[k][i] = (k < na && wvalid && hr0 + i < h1) ? __builtin_nontemporal_load(ap + (int64_t)i * p.W)
+                                                              : 0.0f;
+        }
         int gi = 0;
         while (gi < na) {
             // Longest run of angles gi..gj-1 whose union footprint fits the budget.
@@ -159,12 +172,24 @@ __global__ __launch_bounds__(256) void fwd_strip_kernel(FwdArgs p)
             gj = uniform_int(gj);
 
             __syncthreads();  // everyone is done with the previous footprint
+            // Staging: the footprint as a flat index range; eight loads in flight per thread.
             for (int rr = 0; rr < p.nref; ++rr) {
-                const float *Cs = p.C + ((int64_t)(p.r0 + rr) * p.B + b) * HW;
+                const float *Cs = p.C + ((int64_t)(p.r0 + rr) * p.B + b) * HW + (int64_t)rlo * p.W + clo;
                 float *dst = lds + rr * fsz;
-                for (int r = wave; r < rows; r += kFWaves) {
-                    const float *src = Cs + (int64_t)(rlo + r) * p.W + clo;
-                    for (int cc = lane; cc < cols; cc += 64) dst[r * pitch + cc] = src[cc];
+                const int n = rows * cols;
+                for (int base = threadIdx.x; base < n; base += 8 * 256) {
+                    float v[8];
+                    int off[8];
+#pragma unroll
+                    for (int e = 0; e < 8; ++e) {
+                        const int idx = base + e * 256;
+                        const int r = idx / cols, cc = idx - r * cols;
+                        off[e] = r * pitch + cc;
+                        v[e] = idx < n ? Cs[(int64_t)r * p.W + cc] : 0.0f;
+                    }
+#pragma unroll
+                    for (int e = 0; e < 8; ++e)
+                        if (base + e * 256 < n) dst[off[e]] = v[e];
                 }
             }
             __syncthreads();
@@ -175,11 +200,6 @@ __global__ __launch_bounds__(256) void fwd_strip_kernel(FwdArgs p)
                 const float c = trig[2 * k], s = trig[2 * k + 1];
                 const float bx = fmaf(-s, wc, cy);
                 const float by = fmaf(c, wc, cx);
-                const float *ap = p.atten + ((int64_t)(a0 + k) * p.B + b) * HW + (int64_t)hr0 * p.W + w;
-                float av[kFRows];
-#pragma unroll
-                for (int i = 0; i < kFRows; ++i)
-                    av[i] = (wvalid && hr0 + i < h1) ? __builtin_nontemporal_load(ap + (int64_t)i * p.W) : 0.0f;
                 float sum = 0.0f;
 #pragma unroll
                 for (int i = 0; i < kFRows; ++i) {
@@ -195,7 +215,7 @@ __global__ __launch_bounds__(256) void fwd_strip_kernel(FwdArgs p)
                         const float v10 = lds[li + pitch], v11 = lds[li + pitch + 1];
                         const float top = fmaf(fy, v01 - v00, v00);
                         const float bot = fmaf(fy, v11 - v10, v10);
-                        sum = fmaf(fmaf(fx, bot - top, top), av[i], sum);
+                        sum = fmaf(fmaf(fx, bot - top, top), av[k][i], sum);
                     } else {
                         float val = 0.0f;
                         for (int rr = 0; rr < p.nref; ++rr) {
@@ -206,7 +226,7 @@ __global__ __launch_bounds__(256) void fwd_strip_kernel(FwdArgs p)
                             const float bot = fmaf(fy, v11 - v10, v10);
                             val = fmaf(emv[k * kFRefChunk + rr], fmaf(fx, bot - top, top), val);
                         }
-                        sum = fmaf(val, av[i], sum);
+                        sum = fmaf(val, av[k][i], sum);
                     }
                 }
                 acc[k] += sum;
@@ -242,13 +262,13 @@ __global__ __launch_bounds__(256) void fwd_strip_kernel(FwdArgs p)
 // ---------------------------------------------------------------------------
 // Back-projection (exact transpose) and the fused MLEM update
 // ---------------------------------------------------------------------------
-constexpr int kBI = 32;               // image rows per tile
-constexpr int kBJ = 64;               // image cols per tile
+constexpr int kBI = 31;               // image rows per tile (clamped-sample region: 32 rows)
+constexpr int kBJ = 63;               // image cols per tile (region: 64 cols = one wave)
 constexpr int kBWaves = 4;            // private accumulators per workgroup
-constexpr int kBPitch = kBJ + 3;      // halo of one on each side, odd pitch
+constexpr int kBPitch = kBJ + 2;      // 65: region cols plus the right neighbour (odd)
 constexpr int kBAcc = (kBI + 2) * kBPitch;
 constexpr int kBRefChunk = 2;
-constexpr int kBUnroll = 4;           // 8x8 blocks loaded together
+constexpr int kBUnroll = 4;           // row segments loaded together
 
 enum BwdMode { kBwdWrite = 0, kBwdSens = 1, kBwdUpdate = 2, kBwdAccumulate = 3 };
 
@@ -284,12 +304,12 @@ __device__ __forceinline__ float huber_at(const float *Cs, int i, int j, int H, 
     return g;
 }
 
-// Range of qc with lo <= slope*qc + base(h) < hi for some h of a block row, where
-// base(h) runs between bmin and bmax over the row's h.  Conservative by one.
-__device__ __forceinline__ void slope_interval(float slope, float lo, float hi, float bmin, float bmax,
-                                               float &qmin, float &qmax)
+// Range of qc with lo <= slope*qc + base < hi, intersected into [qmin, qmax];
+// widened by one on each side (the exact test is made per point).
+__device__ __forceinline__ void slope_interval(float slope, float lo, float hi, float base, float &qmin,
+                                               float &qmax)
 {
-    const float L = lo - bmax, U = hi - bmin;  // need L <= slope*qc < U
+    const float L = lo - base, U = hi - base;  // need L <= slope*qc < U
     if (slope > 1e-6f) {
         qmin = fmaxf(qmin, L / slope - 1.0f);
         qmax = fminf(qmax, U / slope + 1.0f);
@@ -321,21 +341,20 @@ __global__ __launch_bounds__(256) void bwd_tile_kernel(BwdArgs p)
     const float xmax = p.H - 1.001f, ymax = p.W - 1.001f;
     const int64_t HW = (int64_t)p.H * p.W;
     constexpr float BIG = 1.0e6f;
-    // Unclamped sample rows whose clamped value lands in [I0-1, I0+T): open on a
-    // side where the clamp folds everything beyond onto the tile.
+    // Unclamped sample rows whose clamped value lands in [I0-1, I0+kBI): open on
+    // a side where the clamp folds everything beyond onto the tile.
     const float lox = I0 <= 1 ? -BIG : (float)(I0 - 1);
     const float hix = I0 + kBI >= p.H - 1 ? BIG : (float)(I0 + kBI);
     const float loy = J0 <= 1 ? -BIG : (float)(J0 - 1);
     const float hiy = J0 + kBJ >= p.W - 1 ? BIG : (float)(J0 + kBJ);
     const bool border = lox < 0.0f || hix > (float)p.H || loy < 0.0f || hiy > (float)p.W;
-    const int lh = lane >> 3, lw = lane & 7;
 
     for (int a = 0; a < p.A; ++a) {
         const float tha = finite_or_zero(p.theta[a]);
         const float c = cosf(tha), s = sinf(tha);
         // backward_emission_batch_kernel (.cu:416-417):
         //   col = c*(q-cx) - s*(h-cy) + cx,  row = s*(q-cx) + c*(h-cy) + cy
-        // Preimage bounding box: invert the rotation at the region's corners.
+        // Preimage rows: invert the rotation at the region's corners.
         float hmin = 3.0e38f, hmax = -3.0e38f;
         {
             const float Xs[2] = {lox - cy, hix - cy};
@@ -357,53 +376,106 @@ __global__ __launch_bounds__(256) void bwd_tile_kernel(BwdArgs p)
 #pragma unroll
         for (int rr = 0; rr < kBRefChunk; ++rr) ew[rr] = rr < p.nref ? p.em[(int64_t)a * p.R + p.r0 + rr] : 0.0f;
 
-        for (int hb = hlo + 8 * wave; hb <= hhi; hb += 8 * kBWaves) {
-            // Column interval of this block row: rows hb..hb+7.
-            const float hcA = (float)hb - cy, hcB = (float)(hb + 7) - cy;
+        // This wave's rows: h = hlo + wave + 4k.  Lane k computes row k's column interval.
+        for (int kb = 0; hlo + wave + 4 * kb <= hhi; kb += 64) {
+            const int hk = hlo + wave + 4 * (kb + lane);
             float qmin = -cx - 1.0f, qmax = (float)(p.W - 1) - cx + 1.0f;
-            slope_interval(s, lox - cy, hix - cy, fminf(c * hcA, c * hcB), fmaxf(c * hcA, c * hcB), qmin, qmax);
-            slope_interval(c, loy - cx, hiy - cx, fminf(-s * hcA, -s * hcB), fmaxf(-s * hcA, -s * hcB), qmin, qmax);
-            const int ql = uniform_int((int)clampf(floorf(qmin + cx), 0.f, (float)p.W));
-            const int qh = uniform_int((int)clampf(ceilf(qmax + cx), -1.f, (float)(p.W - 1)));
-            if (ql > qh) continue;
-            const int h = hb + lh;
-            const float hc = (float)h - cy;
-            const float rowb = fmaf(c, hc, cy);
-            const float colb = fmaf(-s, hc, cx);
-            for (int qb = ql; qb <= qh; qb += 8 * kBUnroll) {
+            {
+                const float hc = (float)hk - cy;
+                slope_interval(s, lox, hix, fmaf(c, hc, cy), qmin, qmax);
+                slope_interval(c, loy, hiy, fmaf(-s, hc, cx), qmin, qmax);
+            }
+            const int qlk = (int)clampf(floorf(qmin + cx), 0.f, (float)p.W);
+            const int qhk = hk <= hhi ? (int)clampf(ceilf(qmax + cx), -1.f, (float)(p.W - 1)) : -1;
+            const int lenk = max(qhk - qlk + 1, 0);
+            const int nrows = min(64, (hhi - (hlo + wave + 4 * kb)) / 4 + 1);
+            // longest row of the batch decides the lane layout: two rows per
+            // instruction (32 lanes each) when every row fits in 32 lanes
+            int maxlen = lenk;
+#pragma unroll
+            for (int off = 32; off >= 1; off >>= 1) maxlen = max(maxlen, __shfl_xor(maxlen, off));
+            maxlen = uniform_int(maxlen);
+            if (maxlen == 0) continue;
+            const bool half = maxlen <= 32;
+            const int rows_per = half ? 2 : 1;
+            const int nch = half ? 1 : (maxlen + 63) / 64;
+            const int lsub = half ? (lane & 31) : lane;
+            const int lrow = half ? (lane >> 5) : 0;
+            const int nitems = ((nrows + rows_per - 1) / rows_per) * nch;
+
+            for (int it0 = 0; it0 < nitems; it0 += kBUnroll) {
                 float av[kBUnroll], rv[kBUnroll], X[kBUnroll], Y[kBUnroll];
                 bool in[kBUnroll];
 #pragma unroll
                 for (int u = 0; u < kBUnroll; ++u) {
-                    const int q = qb + 8 * u + lw;
-                    const float qc = (float)q - cx;
-                    X[u] = fmaf(s, qc, rowb);
-                    Y[u] = fmaf(c, qc, colb);
-                    in[u] = h <= hhi && q <= qh && X[u] >= lox && X[u] < hix && Y[u] >= loy && Y[u] < hiy;
+                    const int it = it0 + u;
+                    const int rowi = (it / nch) * rows_per + lrow;  // row index within the batch
+                    const int ch = it % nch;
+                    const int rsel = min(rowi, 63);
+                    const int ql = __shfl(qlk, rsel), qh = __shfl(qhk, rsel);
+                    const int h = hlo + wave + 4 * (kb + rowi);
+                    const int q = ql + ch * 64 + lsub;
+                    const float hc = (float)h - cy, qc = (float)q - cx;
+                    X[u] = fmaf(s, qc, fmaf(c, hc, cy));
+                    Y[u] = fmaf(c, qc, fmaf(-s, hc, cx));
+                    in[u] = it < nitems && rowi < nrows && q <= qh && X[u] >= lox && X[u] < hix && Y[u] >= loy &&
+                            Y[u] < hiy;
                     av[u] = in[u] ? ap[(int64_t)h * p.W + q] : 0.0f;
                     rv[u] = (in[u] && rp) ? rp[q] : 1.0f;
                 }
+                // Splat.  Float LDS atomics are slow on this hardware, so the accumulator is
+                // updated with plain read-add-write, ordered so that no two lanes touch
+                // one word in the same instruction: along a detector row two points two
+                // apart differ by (2s, 2c), at least 1.41 in one coordinate, so their
+                // floor cells differ -- even-column lanes first, then odd ones, one
+                // corner at a time.  Segments of different rows are >= 4 rows apart and
+                // cannot meet.  Samples moved by the clamp (border tiles only) can
+                // collapse onto one cell; they go through atomics in a third phase.
+                int li[kBUnroll];
+                float wq[kBUnroll][4];
+                bool fast[kBUnroll], slow[kBUnroll], odd[kBUnroll];
 #pragma unroll
                 for (int u = 0; u < kBUnroll; ++u) {
-                    if (!in[u]) continue;
-                    const float rc = border ? clampf(X[u], 0.f, xmax) : X[u];
-                    const float cc = border ? clampf(Y[u], 0.f, ymax) : Y[u];
-                    const int y0 = (int)rc, x0 = (int)cc;
-                    const float dy = rc - (float)y0, dx = cc - (float)x0;
+                    const float Xc = clampf(X[u], 0.f, xmax), Yc = clampf(Y[u], 0.f, ymax);
+                    const bool clamped = border && (Xc != X[u] || Yc != Y[u]);
+                    const int y0 = (int)Xc, x0 = (int)Yc;
+                    const float dy = Xc - (float)y0, dx = Yc - (float)x0;
                     const float v = av[u] * rv[u];
-                    const int li = (y0 - I0 + 1) * kBPitch + (x0 - J0 + 1);
-                    const float b0 = 1.0f - dx, b1 = dx;
+                    li[u] = in[u] ? (y0 - I0 + 1) * kBPitch + (x0 - J0 + 1) : 0;
+                    const float wt = v * (1.0f - dy), wb = v * dy;
+                    wq[u][0] = wt * (1.0f - dx);
+                    wq[u][1] = wt * dx;
+                    wq[u][2] = wb * (1.0f - dx);
+                    wq[u][3] = wb * dx;
+                    fast[u] = in[u] && !clamped;
+                    slow[u] = in[u] && clamped;
+                    odd[u] = (lsub & 1) != 0;
+                }
+                const int offs[4] = {0, 1, kBPitch, kBPitch + 1};
 #pragma unroll
-                    for (int rr = 0; rr < kBRefChunk; ++rr) {
-                        if (rr >= p.nref) break;
-                        const float wv = ew[rr] * v;
-                        const float wt = wv * (1.0f - dy), wb = wv * dy;
-                        float *A_ = acc + rr * kBAcc + li;
-                        atomicAdd(A_, wt * b0);
-                        atomicAdd(A_ + 1, wt * b1);
-                        atomicAdd(A_ + kBPitch, wb * b0);
-                        atomicAdd(A_ + kBPitch + 1, wb * b1);
+                for (int rr = 0; rr < kBRefChunk; ++rr) {
+                    if (rr >= p.nref) break;
+                    float *A_ = acc + rr * kBAcc;
+#pragma unroll
+                    for (int ph = 0; ph < 2; ++ph) {
+#pragma unroll
+                        for (int k = 0; k < 4; ++k) {
+                            float cur[kBUnroll];
+#pragma unroll
+                            for (int u = 0; u < kBUnroll; ++u)
+                                cur[u] = (fast[u] && odd[u] == (ph == 1)) ? A_[li[u] + offs[k]] : 0.0f;
+#pragma unroll
+                            for (int u = 0; u < kBUnroll; ++u)
+                                if (fast[u] && odd[u] == (ph == 1)) A_[li[u] + offs[k]] = fmaf(ew[rr], wq[u][k], cur[u]);
+                            __asm__ volatile("" ::: "memory");
+                        }
                     }
+#pragma unroll
+                    for (int u = 0; u < kBUnroll; ++u)
+                        if (slow[u])
+#pragma unroll
+                            for (int k = 0; k < 4; ++k) atomicAdd(A_ + li[u] + offs[k], ew[rr] * wq[u][k]);
+                    __asm__ volatile("" ::: "memory");
                 }
             }
         }
@@ -417,7 +489,7 @@ __global__ __launch_bounds__(256) void bwd_tile_kernel(BwdArgs p)
         const int64_t sb = ((int64_t)r * p.B + b) * HW;
         for (int ii = wave; ii < kBI; ii += kBWaves) {
             const int i = I0 + ii;
-            if (i >= p.H || j >= p.W) continue;
+            if (i >= p.H || j >= p.W || lane >= kBJ) continue;
             const int li = rr * kBAcc + (ii + 1) * kBPitch + lane + 1;
             const int st = p.nref * kBAcc;
             const float back = ((lds[li] + lds[st + li]) + lds[2 * st + li]) + lds[3 * st + li];
