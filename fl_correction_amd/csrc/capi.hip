@@ -47,6 +47,8 @@ static int check_dims(int64_t A, int64_t B, int64_t R, int64_t H, int64_t W)
                "shape: need A >= 0, B >= 0, R >= 1, H >= 2, W >= 2");
     FL_REQUIRE(H < (1 << 20) && W < (1 << 20) && A < (1 << 24) && B < (1 << 24) && R < 65536,
                "shape: dimension too large");
+    // one slice is addressed with 32-bit byte offsets
+    FL_REQUIRE(H * W < (int64_t)1 << 29, "shape: H*W must be below 2^29");
     return FL_OK;
 }
 

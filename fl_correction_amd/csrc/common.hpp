@@ -59,6 +59,28 @@ __device__ __forceinline__ void rot_sample(int i, int j, int H, int W, float c, 
 
 __device__ __forceinline__ int uniform_int(int v) { return __builtin_amdgcn_readfirstlane(v); }
 
+// Raw buffer resource over n floats at p (CDNA: word 3 = 0x00020000, 32-bit
+// data format).  Loads through it are range-checked by the hardware: a byte
+// offset at or past 4*n reads 0 instead of touching memory.
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t make_rsrc(const float *p, int n)
+{
+    return __builtin_amdgcn_make_buffer_rsrc(const_cast<float *>(p), (short)0, n * 4, 0x00020000);
+}
+
+// Read of a wave-uniform element through the scalar data cache (s_load: counted
+// apart from the vector loads, so it does not wait for loads in flight).  Only
+// for inputs that the kernel does not write.
+__device__ __forceinline__ float scalar_read(const float *p, int64_t i)
+{
+    return ((__attribute__((address_space(4))) const float *)p)[i];
+}
+
+// One float at byte offset off (0 past the end).
+__device__ __forceinline__ float load_f32(__amdgpu_buffer_rsrc_t r, uint32_t off)
+{
+    return __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(r, off, 0, 0));
+}
+
 __device__ __forceinline__ float uniform_float(float v)
 {
     return __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(v)));

@@ -262,13 +262,26 @@ __global__ __launch_bounds__(256, 2) void fwd_strip_kernel(FwdArgs p)
 // ---------------------------------------------------------------------------
 // Back-projection (exact transpose) and the fused MLEM update
 // ---------------------------------------------------------------------------
-constexpr int kBI = 31;               // image rows per tile (clamped-sample region: 32 rows)
-constexpr int kBJ = 63;               // image cols per tile (region: 64 cols = one wave)
+constexpr int kBI = 31;               // image rows per tile (base cells of its samples: 32 rows)
+constexpr int kBJ = 63;               // image cols per tile (base cells: 64 cols)
 constexpr int kBWaves = 4;            // private accumulators per workgroup
-constexpr int kBPitch = kBJ + 2;      // 65: region cols plus the right neighbour (odd)
+constexpr int kBPitch = kBJ + 2;      // 65: base-cell cols plus the right neighbour (odd)
 constexpr int kBAcc = (kBI + 2) * kBPitch;
 constexpr int kBRefChunk = 2;
-constexpr int kBUnroll = 4;           // row segments loaded together
+constexpr int kBSink = 136;           // per-wave sink cells: lane + corner offset <= 63 + 66
+constexpr int kBItems = 32;           // items of 128 samples per enumeration window
+constexpr int kBWin = 128 * kBItems;  // samples per window
+constexpr int kBUnroll = 2;           // items splatted together
+constexpr int kBSlots = 256;          // election table for clamped samples (hash of the cell)
+static_assert(kBItems * 2 == 64, "one mask word per lane");
+static_assert(kBItems % kBUnroll == 0, "items come in whole unroll groups");
+
+// Dynamic LDS of the back-projector: four private accumulators (cell-major,
+// nref values per cell), then the sinks.
+constexpr size_t bwd_lds_bytes(int nref)
+{
+    return sizeof(float) * (size_t)kBWaves * nref * (kBAcc + kBSink);
+}
 
 enum BwdMode { kBwdWrite = 0, kBwdSens = 1, kBwdUpdate = 2, kBwdAccumulate = 3 };
 
@@ -304,195 +317,450 @@ __device__ __forceinline__ float huber_at(const float *Cs, int i, int j, int H, 
     return g;
 }
 
-// Range of qc with lo <= slope*qc + base < hi, intersected into [qmin, qmax];
-// widened by one on each side (the exact test is made per point).
-__device__ __forceinline__ void slope_interval(float slope, float lo, float hi, float base, float &qmin,
-                                               float &qmax)
+// Range of qc with lo <= fmaf(slope, qc, base) < hi, intersected into [qmin, qmax].
+// The real-arithmetic bounds are widened by one plus the distance the rounding
+// of the fma (at most tol in value) can move them, so the range always contains
+// every qc that passes the exact test; the caller tightens it point by point.
+__device__ __forceinline__ void slope_interval(float slope, float lo, float hi, float base, float tol,
+                                               float &qmin, float &qmax)
 {
-    const float L = lo - base, U = hi - base;  // need L <= slope*qc < U
-    if (slope > 1e-6f) {
-        qmin = fmaxf(qmin, L / slope - 1.0f);
-        qmax = fminf(qmax, U / slope + 1.0f);
-    } else if (slope < -1e-6f) {
-        qmin = fmaxf(qmin, U / slope - 1.0f);
-        qmax = fminf(qmax, L / slope + 1.0f);
-    } else if (!(L <= 1e-3f && U > -1e-3f)) {
+    const float L = lo - base, U = hi - base;
+    const float as = fabsf(slope);
+    if (as > 0.0f) {
+        // approximate quotients: their error is far inside the margin m
+        const float r = __frcp_rn(slope);
+        const float m = 1.0f + tol * fabsf(r);
+        const float a = (slope > 0.0f ? L : U) * r, z = (slope > 0.0f ? U : L) * r;
+        qmin = fmaxf(qmin, a - m);
+        qmax = fminf(qmax, z + m);
+    } else if (!(L <= 0.0f && U > 0.0f)) {
         qmin = 1.0f;
         qmax = 0.0f;  // empty
     }
 }
 
+__device__ __forceinline__ int lanes_below(uint64_t mask)
+{
+    return (int)__builtin_amdgcn_mbcnt_hi((uint32_t)(mask >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)mask, 0u));
+}
+
+// Per-wave view of the back-projector's state for one tile.
+struct BwdTile {
+    int I0, J0, b;
+    float lox, hix, loy, hiy;  // region of unclamped sample positions
+    float tol;                 // rounding allowance of one sample coordinate
+    float *lds;                // dynamic LDS: accumulators (cell-major, NR values per cell), sinks
+    int cbase;                 // image cell (y0, x0) -> accumulator cell y0 * kBPitch + x0 + cbase
+    uint32_t *owner;           // election table (clamped samples)
+    uint32_t *mask;            // row starts of the current window, two words per item
+    int4 *rows;                // compacted rows of the current batch
+    int lane, sink;            // sink: this lane's sink cell
+};
+
+// Sample stream of one wave.  Angles a = wave, wave+4, ...; for each angle the
+// detector rows of the tile's preimage in two classes, h = hlo + cls + 2k, 64
+// rows per batch (lane k holds row k).  Each row's in-region samples form one
+// column interval, padded to an even count; a batch's samples, row after row,
+// are numbered f = 0..T-1, cut into windows of kBWin and items of 128: lane l
+// of item n holds the pair f = 128n+2l, 128n+2l+1 (points 0 and 1), always of
+// one row.  A splat instruction uses one point per lane, all of one parity, so
+// its samples lie >= 2 apart on one row or on rows >= 2 apart: >= 1.41 apart in
+// one coordinate, so their base cells differ, and a plain read-add-write, one
+// corner at a time, cannot lose an update (float LDS atomics are far slower on
+// this hardware).  Samples moved by the clamp (border tiles only) can share a
+// cell; they are added afterwards, one lane per cell per round, that lane
+// elected by an integer minimum.
+struct BwdCursor {
+    int a, stage;  // stage 0: set up angle a; 1: set up the batch at rb; 2: inside a window
+    int cls, rb, hlo, hhi;
+    int T, nne, win0, nit, n, kb;
+    int P;    // this lane's row: first sample number (batch setup)
+    bool ne;  // this lane's row is not empty
+    float c, s;
+    bool has_ratio;
+    __amdgpu_buffer_rsrc_t ra, rr;
+    float ew[kBRefChunk];
+};
+
+// One item pair (256 samples, four per lane) between its loads and its splat.
+struct BwdPair {
+    float X[kBUnroll][2], Y[kBUnroll][2], av[kBUnroll][2], rv[kBUnroll][2];
+    bool val[kBUnroll][2];
+    float ew[kBRefChunk];
+};
+
+template <int NR>
+__device__ __forceinline__ void bwd_begin_angle(const BwdArgs &p, const BwdTile &t, BwdCursor &cur)
+{
+    const float cx = (p.W - 1) * 0.5f, cy = (p.H - 1) * 0.5f;
+    const float tha = finite_or_zero(scalar_read(p.theta, cur.a));
+    const float c = cosf(tha), s = sinf(tha);
+    // backward_emission_batch_kernel (.cu:416-417):
+    //   col = c*(q-cx) - s*(h-cy) + cx,  row = s*(q-cx) + c*(h-cy) + cy
+    // Preimage rows: invert the rotation at the region's corners.
+    float hmin = 3.0e38f, hmax = -3.0e38f;
+    const float Xs[2] = {t.lox - cy, t.hix - cy};
+    const float Ys[2] = {t.loy - cx, t.hiy - cx};
+#pragma unroll
+    for (int i = 0; i < 2; ++i)
+#pragma unroll
+        for (int j = 0; j < 2; ++j) {
+            const float hc = c * Xs[i] - s * Ys[j];
+            hmin = fminf(hmin, hc);
+            hmax = fmaxf(hmax, hc);
+        }
+    cur.hlo = uniform_int((int)clampf(floorf(hmin + cy) - 1.f, 0.f, (float)p.H));
+    cur.hhi = uniform_int((int)clampf(ceilf(hmax + cy) + 1.f, -1.f, (float)(p.H - 1)));
+    cur.c = c;
+    cur.s = s;
+    const int64_t sl = (int64_t)cur.a * p.B + t.b;
+    cur.ra = make_rsrc(p.atten + sl * p.H * p.W, p.H * p.W);
+    cur.has_ratio = p.ratio != nullptr;
+    cur.rr = make_rsrc(p.ratio ? p.ratio + sl * p.W : p.atten, p.ratio ? p.W : 0);
+#pragma unroll
+    for (int rr = 0; rr < NR; ++rr) cur.ew[rr] = scalar_read(p.em, (int64_t)cur.a * p.R + p.r0 + rr);
+}
+
+// Column intervals of the batch's rows, their prefix sums and the compacted
+// row table.
+__device__ __forceinline__ void bwd_begin_batch(const BwdArgs &p, const BwdTile &t, BwdCursor &cur)
+{
+    const int lane = t.lane;
+    const float cx = (p.W - 1) * 0.5f, cy = (p.H - 1) * 0.5f;
+    const float c = cur.c, s = cur.s;
+    const int h = cur.rb + 2 * lane;
+    int ql = 0, len = 0;
+    if (h <= cur.hhi) {
+        const float hc = (float)h - cy;
+        const float bx = fmaf(c, hc, cy), by = fmaf(-s, hc, cx);
+        auto in_region = [&](int q) {
+            const float qc = (float)q - cx;
+            const float X = fmaf(s, qc, bx), Y = fmaf(c, qc, by);
+            return X >= t.lox && X < t.hix && Y >= t.loy && Y < t.hiy;
+        };
+        float qmin = -cx, qmax = (float)(p.W - 1) - cx;
+        slope_interval(s, t.lox, t.hix, bx, t.tol, qmin, qmax);
+        slope_interval(c, t.loy, t.hiy, by, t.tol, qmin, qmax);
+        int qa = (int)clampf(floorf(qmin + cx), 0.f, (float)p.W);
+        int qb = (int)clampf(ceilf(qmax + cx), -1.f, (float)(p.W - 1));
+        // Tighten to the exact set: with the same arithmetic as the splat, a
+        // sample passes this test iff its clamped base cell lies in
+        // [I0-1, I0+kBI-1] x [J0-1, J0+kBJ-1].
+        while (qa <= qb && !in_region(qa)) ++qa;
+        while (qa <= qb && !in_region(qb)) --qb;
+        ql = qa;
+        len = max(qb - qa + 1, 0);
+    }
+    const int lp = len + (len & 1);
+    int P = lp;  // inclusive prefix sum over the batch
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+        const int v = __shfl_up(P, d);
+        if (lane >= d) P += v;
+    }
+    cur.T = uniform_int(__shfl(P, 63));
+    P -= lp;
+    const bool ne = len > 0;
+    const uint64_t nem = __ballot(ne);
+    cur.nne = __popcll(nem);
+    // Non-empty rows, compacted: (row * W, first sample - first column, last sample, row - cy).
+    if (ne) t.rows[lanes_below(nem)] = make_int4(h * p.W, P - ql, P + len - 1, __float_as_int((float)h - cy));
+    cur.P = P;
+    cur.ne = ne;
+}
+
+// Row starts (even) of the window at win0, one bit per lane pair: item n's
+// lanes 0-31 / 32-63 in words 2n / 2n+1.
+__device__ __forceinline__ void bwd_begin_window(const BwdTile &t, BwdCursor &cur)
+{
+    t.mask[t.lane] = 0u;
+    if (cur.ne && cur.P >= cur.win0 && cur.P < cur.win0 + kBWin) {
+        const int o = (cur.P - cur.win0) >> 1;
+        atomicOr(&t.mask[o >> 5], 1u << (o & 31));
+    }
+    cur.kb = __popcll(__ballot(cur.ne && cur.P < cur.win0)) - 1;  // row of the sample before win0
+    cur.nit = min(kBItems, (cur.T - cur.win0 + 127) >> 7);
+    cur.n = 0;
+}
+
+// Move to the next item pair of the stream, setting up angles, batches and
+// windows on the way.  False at the end of the stream.
+template <int NR>
+__device__ __forceinline__ bool bwd_next(const BwdArgs &p, const BwdTile &t, BwdCursor &cur)
+{
+    while (true) {
+        if (cur.stage == 2) {
+            cur.n += kBUnroll;
+            if (cur.n < cur.nit) return true;
+            cur.win0 += kBWin;
+            if (cur.win0 < cur.T) {
+                bwd_begin_window(t, cur);
+                return true;
+            }
+            cur.rb += 128;
+            cur.stage = 1;
+        } else if (cur.stage == 1) {
+            if (cur.rb > cur.hhi) {
+                if (cur.cls == 0) {
+                    cur.cls = 1;
+                    cur.rb = cur.hlo + 1;
+                } else {
+                    cur.a += kBWaves;
+                    cur.stage = 0;
+                }
+                continue;
+            }
+            bwd_begin_batch(p, t, cur);
+            if (cur.T == 0) {
+                cur.rb += 128;
+                continue;
+            }
+            cur.win0 = 0;
+            bwd_begin_window(t, cur);
+            cur.stage = 2;
+            return true;
+        } else {
+            if (cur.a >= p.A) {
+                // past the end: leave indices that a dropped gather may still use in range
+                cur.n = 0;
+                cur.nne = max(cur.nne, 1);
+                return false;
+            }
+            bwd_begin_angle<NR>(p, t, cur);
+            cur.cls = 0;
+            cur.rb = cur.hlo;
+            cur.stage = 1;
+        }
+    }
+}
+
+// Positions of the current item pair's samples, and their atten and ratio
+// loads (bounds-checked: an offset past the slice reads 0, so lanes without a
+// sample need no branch; their values go to the sink).
+template <int NR>
+__device__ __forceinline__ void bwd_gather(const BwdArgs &p, const BwdTile &t, BwdCursor &cur, BwdPair &g)
+{
+    const int lane = t.lane;
+    const float cx = (p.W - 1) * 0.5f, cy = (p.H - 1) * 0.5f;
+    const float c = cur.c, s = cur.s;
+#pragma unroll
+    for (int u = 0; u < kBUnroll; ++u) {
+        const uint2 m = *reinterpret_cast<const uint2 *>(&t.mask[(cur.n + u) * 2]);
+        const uint64_t E = (uint64_t)m.x | ((uint64_t)m.y << 32);
+        const int ci = cur.kb + lanes_below(E) + (int)((E >> lane) & 1);
+        cur.kb += __popcll(E);
+        const int f0 = cur.win0 + (cur.n + u) * 128 + 2 * lane;
+        const int4 rw = t.rows[min(max(ci, 0), cur.nne - 1)];
+        const int q = f0 - rw.y;
+        const float qc = (float)q - cx, hc = __int_as_float(rw.w);
+        const float bx = fmaf(c, hc, cy), by = fmaf(-s, hc, cx);
+        g.X[u][0] = fmaf(s, qc, bx);
+        g.Y[u][0] = fmaf(c, qc, by);
+        g.X[u][1] = fmaf(s, qc + 1.0f, bx);
+        g.Y[u][1] = fmaf(c, qc + 1.0f, by);
+        const uint32_t oa = (uint32_t)(rw.x + q) * 4u;
+        g.av[u][0] = load_f32(cur.ra, oa);
+        g.av[u][1] = load_f32(cur.ra, oa + 4u);
+        if (cur.has_ratio) {
+            const uint32_t orat = (uint32_t)q * 4u;
+            g.rv[u][0] = load_f32(cur.rr, orat);
+            g.rv[u][1] = load_f32(cur.rr, orat + 4u);
+        } else {
+            g.rv[u][0] = g.rv[u][1] = 1.0f;
+        }
+        g.val[u][0] = f0 <= rw.z;
+        g.val[u][1] = f0 < rw.z;
+    }
+#pragma unroll
+    for (int rr = 0; rr < NR; ++rr) g.ew[rr] = cur.ew[rr];
+}
+
+// Splat one item pair into the wave's accumulator.  BORDER: the tile touches
+// the image edge, so samples may be clamped (interior tiles have none and skip
+// the clamp entirely).
+template <int NR, bool BORDER>
+__device__ __forceinline__ void bwd_splat(const BwdArgs &p, const BwdTile &t, const BwdPair &g)
+{
+    const float xmax = p.H - 1.001f, ymax = p.W - 1.001f;
+    float *lds = t.lds;
+    int cell[kBUnroll][2];
+    float wq[kBUnroll][2][4];
+    bool fast[kBUnroll][2], slow[kBUnroll][2];
+#pragma unroll
+    for (int u = 0; u < kBUnroll; ++u)
+#pragma unroll
+        for (int pt = 0; pt < 2; ++pt) {
+            float Xc = g.X[u][pt], Yc = g.Y[u][pt];
+            bool cl = false;
+            if (BORDER) {
+                Xc = clampf(Xc, 0.f, xmax);
+                Yc = clampf(Yc, 0.f, ymax);
+                cl = Xc != g.X[u][pt] || Yc != g.Y[u][pt];
+            }
+            // sample positions are >= 0 here, so (int) is floor and fract is the
+            // exact remainder
+            const int y0 = (int)Xc, x0 = (int)Yc;
+            const float dy = __builtin_amdgcn_fractf(Xc), dx = __builtin_amdgcn_fractf(Yc);
+            cell[u][pt] = y0 * kBPitch + x0 + t.cbase;
+            fast[u][pt] = g.val[u][pt] && !cl;
+            slow[u][pt] = g.val[u][pt] && cl;
+            // w*(1-dx)*(1-dy), w*dx*(1-dy), w*(1-dx)*dy, w*dx*dy  (.cu:434-443),
+            // formed as differences: w*(1-dx) = w - w*dx
+            const float v = g.av[u][pt] * g.rv[u][pt];
+            const float w1 = v * dx, w0 = v - w1;
+            const float w01 = w0 * dy, w11 = w1 * dy;
+            wq[u][pt][0] = w0 - w01;
+            wq[u][pt][1] = w1 - w11;
+            wq[u][pt][2] = w01;
+            wq[u][pt][3] = w11;
+        }
+    constexpr int offs[4] = {0, NR, kBPitch * NR, (kBPitch + 1) * NR};
+#pragma unroll
+    for (int pt = 0; pt < 2; ++pt) {
+        float *pa[kBUnroll];
+#pragma unroll
+        for (int u = 0; u < kBUnroll; ++u) pa[u] = lds + (fast[u][pt] ? cell[u][pt] : t.sink) * NR;
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            if constexpr (NR == 1) {
+                float cur[kBUnroll];
+#pragma unroll
+                for (int u = 0; u < kBUnroll; ++u) cur[u] = pa[u][offs[k]];
+#pragma unroll
+                for (int u = 0; u < kBUnroll; ++u) pa[u][offs[k]] = fmaf(g.ew[0], wq[u][pt][k], cur[u]);
+            } else {
+                float2 cur[kBUnroll];
+#pragma unroll
+                for (int u = 0; u < kBUnroll; ++u) cur[u] = *reinterpret_cast<const float2 *>(pa[u] + offs[k]);
+#pragma unroll
+                for (int u = 0; u < kBUnroll; ++u) {
+                    cur[u].x = fmaf(g.ew[0], wq[u][pt][k], cur[u].x);
+                    cur[u].y = fmaf(g.ew[1], wq[u][pt][k], cur[u].y);
+                    *reinterpret_cast<float2 *>(pa[u] + offs[k]) = cur[u];
+                }
+            }
+            // keep corner k's writes before corner k+1's reads
+            __asm__ volatile("" ::: "memory");
+        }
+    }
+    if (BORDER) {
+#pragma unroll
+        for (int pt = 0; pt < 2; ++pt)
+#pragma unroll
+            for (int u = 0; u < kBUnroll; ++u) {
+                bool pend = slow[u][pt];
+                const int slot = cell[u][pt] & (kBSlots - 1);
+                while (__ballot(pend)) {
+                    if (pend) atomicMin(&t.owner[slot], (uint32_t)t.lane);
+                    __asm__ volatile("" ::: "memory");
+                    const bool won = pend && t.owner[slot] == (uint32_t)t.lane;
+                    if (won) {
+                        float *d = lds + cell[u][pt] * NR;
+#pragma unroll
+                        for (int k = 0; k < 4; ++k) {
+#pragma unroll
+                            for (int rr = 0; rr < NR; ++rr) d[offs[k] + rr] = fmaf(g.ew[rr], wq[u][pt][k], d[offs[k] + rr]);
+                            __asm__ volatile("" ::: "memory");
+                        }
+                        t.owner[slot] = 0xffffffffu;
+                    }
+                    __asm__ volatile("" ::: "memory");
+                    pend = pend && !won;
+                }
+            }
+    }
+}
+
+// The wave's whole stream, software-pipelined: the loads of the next item pair
+// are issued before the current pair is splatted (two pair buffers, used in
+// turn, so no register copy waits on a load in flight).  Past the end of the
+// stream the cursor stays on its last window and one more pair is gathered and
+// dropped: the tables it reads are still valid and its loads are
+// bounds-checked.
+template <int NR, bool BORDER>
+__device__ __forceinline__ void bwd_stream(const BwdArgs &p, const BwdTile &t, int wave)
+{
+    BwdCursor cur;
+    cur.a = wave;
+    cur.stage = 0;
+    if (!bwd_next<NR>(p, t, cur)) return;
+    BwdPair g0, g1;
+    bwd_gather<NR>(p, t, cur, g0);
+    while (true) {
+        bool more = bwd_next<NR>(p, t, cur);
+        bwd_gather<NR>(p, t, cur, g1);
+        bwd_splat<NR, BORDER>(p, t, g0);
+        if (!more) break;
+        more = bwd_next<NR>(p, t, cur);
+        bwd_gather<NR>(p, t, cur, g0);
+        bwd_splat<NR, BORDER>(p, t, g1);
+        if (!more) break;
+    }
+}
+
+template <int NR>
 __global__ __launch_bounds__(256) void bwd_tile_kernel(BwdArgs p)
 {
     extern __shared__ float lds[];
-    uint32_t t = xcd_remap(blockIdx.x, gridDim.x);
-    const int tj = t % p.ntj;
-    t /= p.ntj;
-    const int ti = t % p.nti;
-    const int b = t / p.nti;
+    __shared__ uint32_t s_owner[kBWaves][kBSlots];
+    __shared__ __align__(16) uint32_t s_mask[kBWaves][kBItems * 2];
+    __shared__ int4 s_row[kBWaves][64];
 
+    uint32_t id = xcd_remap(blockIdx.x, gridDim.x);
+    const int tj = id % p.ntj;
+    id /= p.ntj;
+    const int ti = id % p.nti;
+    const int b = id / p.nti;
     const int lane = threadIdx.x & 63;
-    const int wave = threadIdx.x >> 6;
-    const int I0 = ti * kBI, J0 = tj * kBJ;
-    float *acc = lds + wave * p.nref * kBAcc;
-    for (int k = lane; k < p.nref * kBAcc; k += 64) acc[k] = 0.0f;
+    const int wave = uniform_int(threadIdx.x >> 6);  // wave-uniform: keeps per-wave state in SGPRs
+    constexpr int nacc = NR * kBAcc;
 
-    const float cx = (p.W - 1) * 0.5f, cy = (p.H - 1) * 0.5f;
-    const float xmax = p.H - 1.001f, ymax = p.W - 1.001f;
-    const int64_t HW = (int64_t)p.H * p.W;
+    BwdTile t;
+    t.I0 = ti * kBI;
+    t.J0 = tj * kBJ;
+    t.b = b;
     constexpr float BIG = 1.0e6f;
-    // Unclamped sample rows whose clamped value lands in [I0-1, I0+kBI): open on
-    // a side where the clamp folds everything beyond onto the tile.
-    const float lox = I0 <= 1 ? -BIG : (float)(I0 - 1);
-    const float hix = I0 + kBI >= p.H - 1 ? BIG : (float)(I0 + kBI);
-    const float loy = J0 <= 1 ? -BIG : (float)(J0 - 1);
-    const float hiy = J0 + kBJ >= p.W - 1 ? BIG : (float)(J0 + kBJ);
-    const bool border = lox < 0.0f || hix > (float)p.H || loy < 0.0f || hiy > (float)p.W;
+    // Unclamped sample positions whose clamped base cell lies in
+    // [I0-1, I0+kBI-1] x [J0-1, J0+kBJ-1]: open on a side where the clamp folds
+    // everything beyond onto the tile.
+    t.lox = t.I0 <= 1 ? -BIG : (float)(t.I0 - 1);
+    t.hix = t.I0 + kBI >= p.H - 1 ? BIG : (float)(t.I0 + kBI);
+    t.loy = t.J0 <= 1 ? -BIG : (float)(t.J0 - 1);
+    t.hiy = t.J0 + kBJ >= p.W - 1 ? BIG : (float)(t.J0 + kBJ);
+    // Largest rounding error of one sample coordinate near an active bound.
+    t.tol = 2.4e-7f * (float)(2 * max(p.H, p.W) + 4);
+    t.lds = lds;
+    t.cbase = wave * kBAcc + (1 - t.I0) * kBPitch + 1 - t.J0;
+    t.owner = s_owner[wave];
+    t.mask = s_mask[wave];
+    t.rows = s_row[wave];
+    t.lane = lane;
+    // Sink words (after all accumulators): lanes with nothing to add in a step
+    // read-modify-write there, each lane its own NR words.
+    t.sink = kBWaves * kBAcc + wave * kBSink + lane;
+    for (int k = lane; k < nacc; k += 64) lds[wave * nacc + k] = 0.0f;
+    for (int k = lane; k < kBSlots; k += 64) t.owner[k] = 0xffffffffu;
+    const bool border = t.lox < 0.0f || t.hix > (float)p.H || t.loy < 0.0f || t.hiy > (float)p.W;
 
-    for (int a = 0; a < p.A; ++a) {
-        const float tha = finite_or_zero(p.theta[a]);
-        const float c = cosf(tha), s = sinf(tha);
-        // backward_emission_batch_kernel (.cu:416-417):
-        //   col = c*(q-cx) - s*(h-cy) + cx,  row = s*(q-cx) + c*(h-cy) + cy
-        // Preimage rows: invert the rotation at the region's corners.
-        float hmin = 3.0e38f, hmax = -3.0e38f;
-        {
-            const float Xs[2] = {lox - cy, hix - cy};
-            const float Ys[2] = {loy - cx, hiy - cx};
-#pragma unroll
-            for (int i = 0; i < 2; ++i)
-#pragma unroll
-                for (int j = 0; j < 2; ++j) {
-                    const float hc = c * Xs[i] - s * Ys[j];
-                    hmin = fminf(hmin, hc);
-                    hmax = fmaxf(hmax, hc);
-                }
-        }
-        const int hlo = uniform_int((int)clampf(floorf(hmin + cy) - 1.f, 0.f, (float)p.H));
-        const int hhi = uniform_int((int)clampf(ceilf(hmax + cy) + 1.f, -1.f, (float)(p.H - 1)));
-        const float *ap = p.atten + ((int64_t)a * p.B + b) * HW;
-        const float *rp = p.ratio ? p.ratio + ((int64_t)a * p.B + b) * p.W : nullptr;
-        float ew[kBRefChunk];
-#pragma unroll
-        for (int rr = 0; rr < kBRefChunk; ++rr) ew[rr] = rr < p.nref ? p.em[(int64_t)a * p.R + p.r0 + rr] : 0.0f;
-
-        // This wave's rows: h = hlo + wave + 4k.  Lane k computes row k's column interval.
-        for (int kb = 0; hlo + wave + 4 * kb <= hhi; kb += 64) {
-            const int hk = hlo + wave + 4 * (kb + lane);
-            float qmin = -cx - 1.0f, qmax = (float)(p.W - 1) - cx + 1.0f;
-            {
-                const float hc = (float)hk - cy;
-                slope_interval(s, lox, hix, fmaf(c, hc, cy), qmin, qmax);
-                slope_interval(c, loy, hiy, fmaf(-s, hc, cx), qmin, qmax);
-            }
-            const int qlk = (int)clampf(floorf(qmin + cx), 0.f, (float)p.W);
-            const int qhk = hk <= hhi ? (int)clampf(ceilf(qmax + cx), -1.f, (float)(p.W - 1)) : -1;
-            const int lenk = max(qhk - qlk + 1, 0);
-            const int nrows = min(64, (hhi - (hlo + wave + 4 * kb)) / 4 + 1);
-            // longest row of the batch decides the lane layout: two rows per
-            // instruction (32 lanes each) when every row fits in 32 lanes
-            int maxlen = lenk;
-#pragma unroll
-            for (int off = 32; off >= 1; off >>= 1) maxlen = max(maxlen, __shfl_xor(maxlen, off));
-            maxlen = uniform_int(maxlen);
-            if (maxlen == 0) continue;
-            const bool half = maxlen <= 32;
-            const int rows_per = half ? 2 : 1;
-            const int nch = half ? 1 : (maxlen + 63) / 64;
-            const int lsub = half ? (lane & 31) : lane;
-            const int lrow = half ? (lane >> 5) : 0;
-            const int nitems = ((nrows + rows_per - 1) / rows_per) * nch;
-
-            for (int it0 = 0; it0 < nitems; it0 += kBUnroll) {
-                float av[kBUnroll], rv[kBUnroll], X[kBUnroll], Y[kBUnroll];
-                bool in[kBUnroll];
-#pragma unroll
-                for (int u = 0; u < kBUnroll; ++u) {
-                    const int it = it0 + u;
-                    const int rowi = (it / nch) * rows_per + lrow;  // row index within the batch
-                    const int ch = it % nch;
-                    const int rsel = min(rowi, 63);
-                    const int ql = __shfl(qlk, rsel), qh = __shfl(qhk, rsel);
-                    const int h = hlo + wave + 4 * (kb + rowi);
-                    const int q = ql + ch * 64 + lsub;
-                    const float hc = (float)h - cy, qc = (float)q - cx;
-                    X[u] = fmaf(s, qc, fmaf(c, hc, cy));
-                    Y[u] = fmaf(c, qc, fmaf(-s, hc, cx));
-                    in[u] = it < nitems && rowi < nrows && q <= qh && X[u] >= lox && X[u] < hix && Y[u] >= loy &&
-                            Y[u] < hiy;
-                    av[u] = in[u] ? ap[(int64_t)h * p.W + q] : 0.0f;
-                    rv[u] = (in[u] && rp) ? rp[q] : 1.0f;
-                }
-                // Splat.  Float LDS atomics are slow on this hardware, so the accumulator is
-                // updated with plain read-add-write, ordered so that no two lanes touch
-                // one word in the same instruction: along a detector row two points two
-                // apart differ by (2s, 2c), at least 1.41 in one coordinate, so their
-                // floor cells differ -- even-column lanes first, then odd ones, one
-                // corner at a time.  Segments of different rows are >= 4 rows apart and
-                // cannot meet.  Samples moved by the clamp (border tiles only) can
-                // collapse onto one cell; they go through atomics in a third phase.
-                int li[kBUnroll];
-                float wq[kBUnroll][4];
-                bool fast[kBUnroll], slow[kBUnroll], odd[kBUnroll];
-#pragma unroll
-                for (int u = 0; u < kBUnroll; ++u) {
-                    const float Xc = clampf(X[u], 0.f, xmax), Yc = clampf(Y[u], 0.f, ymax);
-                    const bool clamped = border && (Xc != X[u] || Yc != Y[u]);
-                    const int y0 = (int)Xc, x0 = (int)Yc;
-                    const float dy = Xc - (float)y0, dx = Yc - (float)x0;
-                    const float v = av[u] * rv[u];
-                    li[u] = in[u] ? (y0 - I0 + 1) * kBPitch + (x0 - J0 + 1) : 0;
-                    const float wt = v * (1.0f - dy), wb = v * dy;
-                    wq[u][0] = wt * (1.0f - dx);
-                    wq[u][1] = wt * dx;
-                    wq[u][2] = wb * (1.0f - dx);
-                    wq[u][3] = wb * dx;
-                    fast[u] = in[u] && !clamped;
-                    slow[u] = in[u] && clamped;
-                    odd[u] = (lsub & 1) != 0;
-                }
-                const int offs[4] = {0, 1, kBPitch, kBPitch + 1};
-#pragma unroll
-                for (int rr = 0; rr < kBRefChunk; ++rr) {
-                    if (rr >= p.nref) break;
-                    float *A_ = acc + rr * kBAcc;
-#pragma unroll
-                    for (int ph = 0; ph < 2; ++ph) {
-#pragma unroll
-                        for (int k = 0; k < 4; ++k) {
-                            float cur[kBUnroll];
-#pragma unroll
-                            for (int u = 0; u < kBUnroll; ++u)
-                                cur[u] = (fast[u] && odd[u] == (ph == 1)) ? A_[li[u] + offs[k]] : 0.0f;
-#pragma unroll
-                            for (int u = 0; u < kBUnroll; ++u)
-                                if (fast[u] && odd[u] == (ph == 1)) A_[li[u] + offs[k]] = fmaf(ew[rr], wq[u][k], cur[u]);
-                            __asm__ volatile("" ::: "memory");
-                        }
-                    }
-#pragma unroll
-                    for (int u = 0; u < kBUnroll; ++u)
-                        if (slow[u])
-#pragma unroll
-                            for (int k = 0; k < 4; ++k) atomicAdd(A_ + li[u] + offs[k], ew[rr] * wq[u][k]);
-                    __asm__ volatile("" ::: "memory");
-                }
-            }
-        }
-    }
+    // Wave w takes angles w, w+4, ... into its own accumulator.
+    if (border)
+        bwd_stream<NR, true>(p, t, wave);
+    else
+        bwd_stream<NR, false>(p, t, wave);
     __syncthreads();
 
     // Sum the four private accumulators in a fixed order and write the tile (lane = column).
+    const int I0 = t.I0, J0 = t.J0;
+    const int64_t HW = (int64_t)p.H * p.W;
     const int j = J0 + lane;
-    for (int rr = 0; rr < p.nref; ++rr) {
+    for (int rr = 0; rr < NR; ++rr) {
         const int r = p.r0 + rr;
         const int64_t sb = ((int64_t)r * p.B + b) * HW;
         for (int ii = wave; ii < kBI; ii += kBWaves) {
             const int i = I0 + ii;
             if (i >= p.H || j >= p.W || lane >= kBJ) continue;
-            const int li = rr * kBAcc + (ii + 1) * kBPitch + lane + 1;
-            const int st = p.nref * kBAcc;
-            const float back = ((lds[li] + lds[st + li]) + lds[2 * st + li]) + lds[3 * st + li];
+            const int li = ((ii + 1) * kBPitch + lane + 1) * NR + rr;
+            const float back = ((lds[li] + lds[nacc + li]) + lds[2 * nacc + li]) + lds[3 * nacc + li];
             const int64_t o = sb + (int64_t)i * p.W + j;
             if (p.mode == kBwdWrite) {
                 p.out[o] = back;
@@ -573,8 +841,10 @@ int launch_backward(const float *atten, const float *ratio, const float *em, con
 {
     static bool attr_set = false;
     if (!attr_set) {
-        FL_HIP_CHECK(hipFuncSetAttribute((const void *)bwd_tile_kernel, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                         (int)(sizeof(float) * kBWaves * kBRefChunk * kBAcc)));
+        FL_HIP_CHECK(hipFuncSetAttribute((const void *)bwd_tile_kernel<1>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                         (int)bwd_lds_bytes(1)));
+        FL_HIP_CHECK(hipFuncSetAttribute((const void *)bwd_tile_kernel<2>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                         (int)bwd_lds_bytes(2)));
         attr_set = true;
     }
     BwdArgs p{};
@@ -600,8 +870,10 @@ int launch_backward(const float *atten, const float *ratio, const float *em, con
     for (int64_t r0 = 0; r0 < R; r0 += kBRefChunk) {
         p.r0 = (int)r0;
         p.nref = (int)std::min<int64_t>(kBRefChunk, R - r0);
-        const size_t shmem = sizeof(float) * (size_t)kBWaves * p.nref * kBAcc;
-        hipLaunchKernelGGL(bwd_tile_kernel, dim3((uint32_t)nwg), dim3(256), shmem, stream, p);
+        if (p.nref == 2)
+            hipLaunchKernelGGL(bwd_tile_kernel<2>, dim3((uint32_t)nwg), dim3(256), bwd_lds_bytes(2), stream, p);
+        else
+            hipLaunchKernelGGL(bwd_tile_kernel<1>, dim3((uint32_t)nwg), dim3(256), bwd_lds_bytes(1), stream, p);
         FL_LAUNCH_CHECK();
     }
     return FL_OK;
