@@ -81,6 +81,13 @@ __device__ __forceinline__ float load_f32(__amdgpu_buffer_rsrc_t r, uint32_t off
     return __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(r, off, 0, 0));
 }
 
+// Same, at byte offset voff + soff (soff wave-uniform), for data read once:
+// marked non-temporal so that it does not evict reused lines from the caches.
+__device__ __forceinline__ float load_f32_stream(__amdgpu_buffer_rsrc_t r, uint32_t voff, uint32_t soff)
+{
+    return __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(r, voff, soff, 2));
+}
+
 __device__ __forceinline__ float uniform_float(float v)
 {
     return __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(v)));

@@ -96,6 +96,7 @@ __device__ __forceinline__ Box fwd_footprint(float c, float s, int h0, int h1, i
     return b;
 }
 
+template <int NR>
 __global__ __launch_bounds__(256, 2) void fwd_strip_kernel(FwdArgs p)
 {
     extern __shared__ float lds[];
@@ -109,7 +110,7 @@ __global__ __launch_bounds__(256, 2) void fwd_strip_kernel(FwdArgs p)
     const int b = t / p.n_groups;
 
     const int lane = threadIdx.x & 63;
-    const int wave = threadIdx.x >> 6;
+    const int wave = uniform_int(threadIdx.x >> 6);
     const int w0 = tw * kFW, w1 = min(w0 + kFW, p.W);
     const int w = w0 + lane;
     const bool wvalid = w < p.W;
@@ -130,7 +131,7 @@ __global__ __launch_bounds__(256, 2) void fwd_strip_kernel(FwdArgs p)
     }
     if (threadIdx.x < kFG * kFRefChunk) {
         const int k = threadIdx.x / kFRefChunk, rr = threadIdx.x % kFRefChunk;
-        emv[threadIdx.x] = (k < na && rr < p.nref) ? p.em[(int64_t)(a0 + k) * p.R + p.r0 + rr] : 0.0f;
+        emv[threadIdx.x] = (k < na && rr < NR) ? p.em[(int64_t)(a0 + k) * p.R + p.r0 + rr] : 0.0f;
     }
     __syncthreads();
 
@@ -143,85 +144,111 @@ __global__ __launch_bounds__(256, 2) void fwd_strip_kernel(FwdArgs p)
         const int hr0 = h0 + wave * kFRows;
         // This chunk's atten rows for all angles of the group, issued before the
         // footprint staging so that their latency overlaps it.
+        // Bounds-checked buffer loads: rows past the image and lanes past the
+        // last column (offset moved out of range) read 0; the row offset is a
+        // scalar operand, so no vector address arithmetic per load.
         float av[kFG][kFRows];
+        const uint32_t voff = wvalid ? (uint32_t)(hr0 * p.W + w) * 4u : 0x80000000u;
 #pragma unroll
         for (int k = 0; k < kFG; ++k) {
-            const float *ap = p.atten + ((int64_t)(a0 + min(k, na - 1)) * p.B + b) * HW + (int64_t)hr0 * p.W + w;
+            const __amdgpu_buffer_rsrc_t ra =
+                make_rsrc(p.atten + ((int64_t)(a0 + min(k, na - 1)) * p.B + b) * HW, k < na ? (int)HW : 0);
 #pragma unroll
-            for (int i = 0; i < kFRows; ++i)
-                av[k][i] = (k < na && wvalid && hr0 + i < h1) ? __builtin_nontemporal_load(ap + (int64_t)i * p.W)
-                                                              : 0.0f;
+            for (int i = 0; i < kFRows; ++i) av[k][i] = load_f32_stream(ra, voff, (uint32_t)(i * p.W) * 4u);
+        }
+        // Rows past the image edge sample the last valid row (times atten 0).
+        float hcs[kFRows];
+#pragma unroll
+        for (int i = 0; i < kFRows; ++i) hcs[i] = (float)min(hr0 + i, h1 - 1) - cy;
+        // Footprint of every angle of the group, lane k computing angle k's box.
+        Box mine{0, 0, 0, 0};
+        {
+            const int k = min(lane, kFG - 1);
+            mine = fwd_footprint(trig[2 * k], trig[2 * k + 1], h0, h1, w0, w1, cx, cy, xmax, ymax);
         }
         int gi = 0;
         while (gi < na) {
             // Longest run of angles gi..gj-1 whose union footprint fits the budget.
-            Box u = fwd_footprint(trig[2 * gi], trig[2 * gi + 1], h0, h1, w0, w1, cx, cy, xmax, ymax);
+            int rlo = __shfl(mine.rlo, gi), rhi = __shfl(mine.rhi, gi);
+            int clo = __shfl(mine.clo, gi), chi = __shfl(mine.chi, gi);
             int gj = gi + 1;
             while (gj < na) {
-                Box nb = fwd_footprint(trig[2 * gj], trig[2 * gj + 1], h0, h1, w0, w1, cx, cy, xmax, ymax);
-                Box m{min(u.rlo, nb.rlo), max(u.rhi, nb.rhi), min(u.clo, nb.clo), max(u.chi, nb.chi)};
-                if ((m.rhi - m.rlo + 1) * ((m.chi - m.clo + 1) | 1) * p.nref > kFLds) break;
-                u = m;
+                const int mrlo = min(rlo, __shfl(mine.rlo, gj)), mrhi = max(rhi, __shfl(mine.rhi, gj));
+                const int mclo = min(clo, __shfl(mine.clo, gj)), mchi = max(chi, __shfl(mine.chi, gj));
+                if ((mrhi - mrlo + 1) * ((mchi - mclo + 1) | 1) * NR > kFLds) break;
+                rlo = mrlo;
+                rhi = mrhi;
+                clo = mclo;
+                chi = mchi;
                 ++gj;
             }
-            const int rlo = uniform_int(u.rlo), clo = uniform_int(u.clo);
-            const int rows = uniform_int(u.rhi) - rlo + 1;
-            const int cols = uniform_int(u.chi) - clo + 1;
+            rlo = uniform_int(rlo);
+            clo = uniform_int(clo);
+            const int rows = uniform_int(rhi) - rlo + 1;
+            const int cols = uniform_int(chi) - clo + 1;
             const int pitch = cols | 1;
             const int fsz = rows * pitch;
             gj = uniform_int(gj);
 
             __syncthreads();  // everyone is done with the previous footprint
-            // Staging: the footprint as a flat index range; eight loads in flight per thread.
-            for (int rr = 0; rr < p.nref; ++rr) {
+            // Staging: wave w takes footprint rows w, w+4, ..., lanes the columns;
+            // eight loads in flight per lane.
+#pragma unroll
+            for (int rr = 0; rr < NR; ++rr) {
                 const float *Cs = p.C + ((int64_t)(p.r0 + rr) * p.B + b) * HW + (int64_t)rlo * p.W + clo;
                 float *dst = lds + rr * fsz;
-                const int n = rows * cols;
-                for (int base = threadIdx.x; base < n; base += 8 * 256) {
-                    float v[8];
-                    int off[8];
+                for (int r0 = wave; r0 < rows; r0 += 4 * kFWaves) {
+                    float v[4][2];
 #pragma unroll
-                    for (int e = 0; e < 8; ++e) {
-                        const int idx = base + e * 256;
-                        const int r = idx / cols, cc = idx - r * cols;
-                        off[e] = r * pitch + cc;
-                        v[e] = idx < n ? Cs[(int64_t)r * p.W + cc] : 0.0f;
-                    }
+                    for (int e = 0; e < 4; ++e)
 #pragma unroll
-                    for (int e = 0; e < 8; ++e)
-                        if (base + e * 256 < n) dst[off[e]] = v[e];
+                        for (int k = 0; k < 2; ++k) {
+                            const int r = r0 + e * kFWaves, cc = lane + 64 * k;
+                            v[e][k] = (r < rows && cc < cols) ? Cs[r * p.W + cc] : 0.0f;
+                        }
+#pragma unroll
+                    for (int e = 0; e < 4; ++e)
+#pragma unroll
+                        for (int k = 0; k < 2; ++k) {
+                            const int r = r0 + e * kFWaves, cc = lane + 64 * k;
+                            if (r < rows && cc < cols) dst[r * pitch + cc] = v[e][k];
+                        }
                 }
             }
             __syncthreads();
 
+            // Footprint cell (x0, y0) is LDS word x0 * pitch + y0 - fbase.
+            const char *fb = reinterpret_cast<const char *>(lds) - 4 * (rlo * pitch + clo);
+            const uint32_t pitch4 = 4u * (uint32_t)pitch;
 #pragma unroll
             for (int k = 0; k < kFG; ++k) {
                 if (k < gi || k >= gj) continue;  // wave-uniform
+                // sample position of this lane's column at row h: (c*(h-cy) + bx, s*(h-cy) + by)
                 const float c = trig[2 * k], s = trig[2 * k + 1];
-                const float bx = fmaf(-s, wc, cy);
-                const float by = fmaf(c, wc, cx);
+                const float bx = fmaf(-s, wc, cy), by = fmaf(c, wc, cx);
                 float sum = 0.0f;
 #pragma unroll
                 for (int i = 0; i < kFRows; ++i) {
-                    // Rows past the image edge sample the last valid row (times atten 0).
-                    const float hc = (float)min(hr0 + i, h1 - 1) - cy;
-                    const float x = clampf(fmaf(c, hc, bx), 0.f, xmax);
-                    const float y = clampf(fmaf(s, hc, by), 0.f, ymax);
+                    const float x = clampf(fmaf(c, hcs[i], bx), 0.f, xmax);
+                    const float y = clampf(fmaf(s, hcs[i], by), 0.f, ymax);
+                    // positions are >= 0: (int) is floor, fract the exact remainder
                     const int x0 = (int)x, y0 = (int)y;
-                    const float fx = x - (float)x0, fy = y - (float)y0;
-                    const int li = (x0 - rlo) * pitch + (y0 - clo);
-                    if (p.nref == 1) {
-                        const float v00 = lds[li], v01 = lds[li + 1];
-                        const float v10 = lds[li + pitch], v11 = lds[li + pitch + 1];
+                    const float fx = __builtin_amdgcn_fractf(x), fy = __builtin_amdgcn_fractf(y);
+                    const float *f =
+                        reinterpret_cast<const float *>(fb + (__umul24((uint32_t)x0, pitch4) + 4u * (uint32_t)y0));
+                    if constexpr (NR == 1) {
+                        const float v00 = f[0], v01 = f[1];
+                        const float v10 = f[pitch], v11 = f[pitch + 1];
                         const float top = fmaf(fy, v01 - v00, v00);
                         const float bot = fmaf(fy, v11 - v10, v10);
                         sum = fmaf(fmaf(fx, bot - top, top), av[k][i], sum);
                     } else {
                         float val = 0.0f;
-                        for (int rr = 0; rr < p.nref; ++rr) {
-                            const float *f = lds + rr * fsz;
-                            const float v00 = f[li], v01 = f[li + 1];
-                            const float v10 = f[li + pitch], v11 = f[li + pitch + 1];
+#pragma unroll
+                        for (int rr = 0; rr < NR; ++rr) {
+                            const float *fr = f + rr * fsz;
+                            const float v00 = fr[0], v01 = fr[1];
+                            const float v10 = fr[pitch], v11 = fr[pitch + 1];
                             const float top = fmaf(fy, v01 - v00, v00);
                             const float bot = fmaf(fy, v11 - v10, v10);
                             val = fmaf(emv[k * kFRefChunk + rr], fmaf(fx, bot - top, top), val);
@@ -802,7 +829,9 @@ int launch_forward(const float *atten, const float *C, const float *em, const fl
     static bool attr_set = false;
     const size_t shmem = sizeof(float) * kFLds;
     if (!attr_set) {
-        FL_HIP_CHECK(hipFuncSetAttribute((const void *)fwd_strip_kernel, hipFuncAttributeMaxDynamicSharedMemorySize,
+        FL_HIP_CHECK(hipFuncSetAttribute((const void *)fwd_strip_kernel<1>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                         (int)shmem));
+        FL_HIP_CHECK(hipFuncSetAttribute((const void *)fwd_strip_kernel<2>, hipFuncAttributeMaxDynamicSharedMemorySize,
                                          (int)shmem));
         attr_set = true;
     }
@@ -829,7 +858,10 @@ int launch_forward(const float *atten, const float *C, const float *em, const fl
         const bool ratio = mode == kFwdRatio;
         p.mode = first ? (last && ratio ? kFwdRatio : kFwdWrite)
                        : (last && ratio ? kFwdAccumulateRatio : kFwdAccumulate);
-        hipLaunchKernelGGL(fwd_strip_kernel, dim3((uint32_t)nwg), dim3(256), shmem, stream, p);
+        if (p.nref == 2)
+            hipLaunchKernelGGL(fwd_strip_kernel<2>, dim3((uint32_t)nwg), dim3(256), shmem, stream, p);
+        else
+            hipLaunchKernelGGL(fwd_strip_kernel<1>, dim3((uint32_t)nwg), dim3(256), shmem, stream, p);
         FL_LAUNCH_CHECK();
     }
     return FL_OK;
