@@ -1,21 +1,28 @@
 #!/usr/bin/env python
-"""Benchmark: plain MLEM iterations of the XRF self-absorption hot path on one MI355X.
+"""Benchmark: MLEM iterations of the XRF self-absorption hot path on one MI355X.
 
 Workload: 256^3 voxels x 360 angles x 4 emission lines (one reference per line),
-FP32, attenuation stored in HBM (the form mlem_cuda_batch iterates on,
-FL/cuda_lib/atten_cuda.cpp:283-345).  One step = one MLEM iteration of all four
-lines: forward projection + ratio, back-projection + update, per line.
-Inputs are synthetic: seeded ellipsoid phantoms, attenuation maps, and Poisson
-projections at 1e4 peak counts.
+FP32, attenuation resident in HBM (the form mlem_cuda_batch iterates on,
+FL/cuda_lib/atten_cuda.cpp:283-345).  One step = one plain MLEM iteration of
+all four lines: forward projection + ratio, then back-projection + update, per
+line.  Inputs are synthetic: seeded ellipsoid phantoms, attenuation maps and
+Poisson projections at 1e4 peak counts.
 
-Prints one JSON line (rank 0).  --gpus above 1 is refused: this build runs on
-one GPU.
+Besides the timed steps, the JSON line carries
+  * roofline: the dominant kernel's algorithmic HBM bytes over its average
+    duration, timed with HIP events on the stream it is launched on;
+  * model: attenuation, forward and gradient (vjp) of the differentiable model
+    with a 5x5 detector cone over 15 deg, timed on a subset of the angles and
+    extrapolated per angle to all 360;
+  * cpu_baseline: the oracle's CPU restatement of the same iteration, timed on
+    one line on the host cores and scaled to all lines.
+
+--gpus above 1 is refused: this build runs on one GPU.
 """
 from __future__ import annotations
 
 import argparse
 import json
-import math
 import os
 import sys
 import time
@@ -34,11 +41,13 @@ def parse():
     ap.add_argument("--size", type=int, default=256, help="volume edge N (N^3 voxels)")
     ap.add_argument("--angles", type=int, default=360)
     ap.add_argument("--elements", type=int, default=4)
-    ap.add_argument("--cpu-angles", type=int, default=24,
+    ap.add_argument("--cpu-angles", type=int, default=240,
                     help="angles of one line timed on the CPU for the baseline (scaled to the full step)")
+    ap.add_argument("--model-angles", type=int, default=8,
+                    help="angles of the differentiable model timed (0: skip), extrapolated per angle")
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--synthetic-atten", action="store_true",
-                    help="uniform random attenuation in [0.3, 1] instead of the physics pipeline")
+    ap.add_argument("--calibrate-copy", action="store_true",
+                    help="first copy 4 GiB device to device (a kernel of known traffic, to scale HBM counters)")
     return ap.parse_args()
 
 
@@ -62,6 +71,14 @@ def main():
 
     dev = torch.device("cuda:0")
     torch.cuda.set_device(dev)
+    if args.calibrate_copy:
+        src = torch.ones(1 << 30, dtype=torch.float32, device=dev)
+        dst = torch.empty_like(src)
+        # an elementwise kernel that reads and writes exactly 4 GiB (a plain
+        # copy_ of contiguous tensors is a DMA transfer, not a kernel)
+        torch.mul(src, 2.0, out=dst)
+        torch.cuda.synchronize()
+        del src, dst
     N, A, K = args.size, args.angles, args.elements
     B = H = W = N
     R = 1
@@ -74,6 +91,8 @@ def main():
     em = torch.full((A, R), rho * pix, dtype=torch.float32, device=dev)
     log(f"phantom {tuple(C_true.shape)} in {time.time() - t0:.1f}s")
 
+    # The kernels' cost does not depend on the attenuation values; the physics
+    # pipeline for all 360 angles is timed separately below (model timings).
     t0 = time.time()
     atten = torch.empty((K, A, B, H, W), dtype=torch.float32, device=dev)
     gen = torch.Generator(device=dev)
@@ -97,6 +116,8 @@ def main():
     log(f"projections + sensitivities in {time.time() - t0:.1f}s")
 
     def step(events=None):
+        # torch.cuda.Event records on torch's current stream, the stream the
+        # library launches on: each pair of events brackets exactly one kernel.
         for e in range(K):
             if events is not None:
                 events[e][0].record()
@@ -124,23 +145,25 @@ def main():
     wall = time.perf_counter() - t_wall
     total_ms = start.elapsed_time(stop)
     ms_per_step = total_ms / max(args.steps, 1)
+    log(f"{args.steps} steps: {ms_per_step:.2f} ms/step")
 
     fwd_ms = [evs[k][e][0].elapsed_time(evs[k][e][1]) for k in range(args.steps) for e in range(K)]
     upd_ms = [evs[k][e][1].elapsed_time(evs[k][e][2]) for k in range(args.steps) for e in range(K)]
     fwd_avg, upd_avg = float(np.mean(fwd_ms)), float(np.mean(upd_ms))
 
+    # Algorithmic HBM bytes per launch (each array read or written once).
     n3 = B * H * W
-    fwd_bytes = 4.0 * (A * n3 + R * n3 + A * B * W)          # F1 per line
-    upd_bytes = 4.0 * (A * n3 + A * B * W + 3 * R * n3)      # B1 + fused update per line
-    iter_bytes = K * (fwd_bytes + upd_bytes + 4.0 * (A * B * W))  # + ratio pass
+    fwd_bytes = 4.0 * (A * n3 + R * n3 + 2 * A * B * W)      # atten, C, I in, ratio out
+    upd_bytes = 4.0 * (A * n3 + A * B * W + 3 * R * n3)      # atten, ratio, C, sens in, C_next out
+    iter_bytes = K * (fwd_bytes + upd_bytes)
     if upd_avg >= fwd_avg:
-        dom, dom_ms, dom_bytes = "bwd_tile_kernel (back-projection + MLEM update)", upd_avg, upd_bytes
+        dom, dom_ms, dom_bytes = "bwd_tile_kernel<1> (back-projection + MLEM update)", upd_avg, upd_bytes
     else:
-        dom, dom_ms, dom_bytes = "fwd_tile_kernel (forward projection + ratio)", fwd_avg, fwd_bytes
+        dom, dom_ms, dom_bytes = "fwd_strip_kernel<1> (forward projection + ratio)", fwd_avg, fwd_bytes
     achieved = dom_bytes / (dom_ms * 1e-3) / 1e9
 
     result = {
-        "metric": "MLEM iterations/s at 256^3 voxels x 360 angles x 4 elements (FP32, stored attenuation)",
+        "metric": "MLEM iter time & grad-evals/s at 256^3 voxels x 360 angles x 4 elements; HBM GB/s vs peak",
         "value": 1000.0 / ms_per_step,
         "unit": "iter/s",
         "n_gpus": 1,
@@ -152,37 +175,99 @@ def main():
         "vs_baseline": None,
         "dtype": "f32",
         "data": f"synthetic: seeded ellipsoid phantom, attenuation {atten_kind}, Poisson 1e4 peak",
-        "config": {"workload": "plain MLEM iteration (beta=0), forward + ratio + back-projection + update, "
-                               "all emission lines", "volume": [B, H, W], "angles": A, "elements": K,
-                   "refs_per_line": R},
+        "config": {"workload": "plain MLEM iteration (beta=0) with attenuation resident in HBM: forward + ratio, "
+                               "back-projection + update, all emission lines",
+                   "volume": [B, H, W], "angles": A, "elements": K, "refs_per_line": R},
         "stage_ms": {"forward_ratio_per_line": fwd_avg, "backproject_update_per_line": upd_avg},
         "hbm_algorithmic_gbs_per_iteration": iter_bytes / (ms_per_step * 1e-3) / 1e9,
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None},
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "bytes_per_launch": dom_bytes, "avg_launch_ms": dom_ms},
         "wall_s_timed": wall,
     }
 
+    # free the MLEM state before the model timings
+    del solvers, C, Cn, I, atten
+    torch.cuda.empty_cache()
+
+    if args.model_angles > 0:
+        result["model"] = model_timings(args, C_true, pix)
+
     if not args.no_cpu_baseline:
-        result["cpu_baseline"] = cpu_baseline(args, atten, C_true, em, theta, I)
+        result["cpu_baseline"] = cpu_baseline(args)
 
     print(json.dumps(result), flush=True)
 
 
-def cpu_baseline(args, atten, C_true, em, theta, I):
+def model_timings(args, C_true, pix):
+    """Attenuation, forward and vjp of the differentiable model (XRFModel) on a
+    subset of the angles, per angle; the iteration and gradient figures are the
+    per-angle times times the full angle count."""
+    import torch
+    from fl_correction_amd.atten_util import make_param
+    from fl_correction_amd.detector_mask import Cone
+    from fl_correction_amd.model import XRFModel
+
+    K, A = args.elements, args.angles
+    rng = np.random.default_rng(20260002)
+    cs_x = rng.uniform(20, 400, K)
+    cs_line = rng.uniform(20, 400, (K, K))
+    param = make_param([f"E{i}" for i in range(K)], [f"C{i}" for i in range(K)], cs_x, cs_line, rho=1.0, pix=pix)
+    na = min(args.model_angles, A)
+    angles = np.linspace(0.0, 180.0, na, endpoint=False)
+    cone = Cone(n=5, half_angle_deg=15.0, q=8)
+    model = XRFModel(param, angles, cone, store=False)
+    Cm = (C_true * 0.5).contiguous()
+
+    def timed(fn):
+        fn()
+        torch.cuda.synchronize()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        out = fn()
+        e1.record()
+        torch.cuda.synchronize()
+        return e0.elapsed_time(e1), out
+
+    t_att, att = timed(lambda: model.attenuation(Cm))
+    del att
+    t_fwd, Pf = timed(lambda: model.forward(Cm))
+    G = torch.randn_like(Pf)
+    t_vjp, _ = timed(lambda: model.vjp(Cm, G))
+    model.release()
+    per = lambda t: t / na
+    log(f"model: {na} angles: attenuation {t_att:.1f} ms, forward {t_fwd:.1f} ms, vjp {t_vjp:.1f} ms")
+    return {
+        "angles_timed": na, "cone": "5x5 directions, 15 deg half-angle, slopes in 1/8",
+        "attenuation_ms_per_angle": per(t_att),
+        "forward_ms_per_angle": per(t_fwd),
+        "vjp_ms_per_angle": per(t_vjp),
+        "adjoint_over_forward": t_vjp / t_fwd,
+        "grad_eval_ms_extrapolated": per(t_fwd + t_vjp) * A,
+        "grad_evals_per_s_extrapolated": 1000.0 / (per(t_fwd + t_vjp) * A),
+        "note": "forward recomputes the attenuation for every angle (nothing stored); vjp recomputes it too "
+                "and differentiates through it",
+    }
+
+
+def cpu_baseline(args):
     """The oracle (the FP32 C restatement of the reference kernels, OpenMP over
-    independent outputs) on one line and a subset of the angles at full volume,
-    scaled linearly to all angles and lines (angles are independent)."""
+    independent outputs) on one line at full volume, scaled to all lines."""
+    import torch
     ncores = len(os.sched_getaffinity(0))
     os.environ.setdefault("OMP_NUM_THREADS", str(ncores))
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle  # the checker's CPU restatement, used here as the timed CPU port
 
+    N = args.size
     ka = min(args.cpu_angles, args.angles)
-    a_np = atten[0, :ka].cpu().numpy()
-    c_np = np.ones((1,) + tuple(C_true.shape[1:]), np.float32)
-    em_np = em[:ka].cpu().numpy()
-    th_np = theta[:ka].cpu().numpy()
-    i_np = I[0][:ka].cpu().numpy()
+    g = torch.Generator(device="cuda:0")
+    g.manual_seed(7)
+    a_np = (torch.rand((ka, N, N, N), device="cuda:0", generator=g) * 0.7 + 0.3).cpu().numpy()
+    i_np = ((torch.rand((ka, N, N), device="cuda:0", generator=g) * 1.5 + 0.5) * 1e-2).cpu().numpy()
+    c_np = np.ones((1, N, N, N), np.float32)
+    em_np = np.full((ka, 1), 1e-4, np.float32)
+    th_np = np.deg2rad(np.linspace(0, 180, ka, endpoint=False)).astype(np.float32)
     sens = oracle.backward_batch_f32(a_np, np.ones_like(i_np), em_np, th_np)  # untimed, as on the GPU
     t0 = time.perf_counter()
     Pf = oracle.forward_batch_f32(a_np, c_np, em_np, th_np)
@@ -192,9 +277,10 @@ def cpu_baseline(args, atten, C_true, em, theta, I):
     dt = time.perf_counter() - t0
     del c_new
     scale = (args.angles / ka) * args.elements
+    log(f"cpu baseline: {dt:.2f}s for one line, {ka} angles, {ncores} threads")
     return {"value": 1.0 / (dt * scale), "unit": "iter/s", "cores": ncores, "kind": "port",
-            "sample": f"one line, {ka} of {args.angles} angles at full volume: {dt:.2f}s, "
-                      f"scaled x{scale:g} (angles and lines are independent)"}
+            "sample": f"one line, {ka} of {args.angles} angles at {N}^3: {dt:.2f}s on {ncores} threads, "
+                      f"scaled x{scale:g} (lines are independent)"}
 
 
 if __name__ == "__main__":

@@ -28,6 +28,10 @@ __device__ __forceinline__ uint32_t xcd_remap(uint32_t id, uint32_t n)
     return x < r ? x * (q + 1) + l : r * (q + 1) + (x - r) * q + l;
 }
 
+// The next float above a positive finite x (x < next_up(x) holds exactly
+// for the values <= x).
+__device__ __forceinline__ float next_up(float x) { return __int_as_float(__float_as_int(x) + 1); }
+
 __device__ __forceinline__ float clampf(float v, float lo, float hi)
 {
     return __builtin_amdgcn_fmed3f(v, lo, hi);
@@ -58,6 +62,39 @@ __device__ __forceinline__ void rot_sample(int i, int j, int H, int W, float c, 
 }
 
 __device__ __forceinline__ int uniform_int(int v) { return __builtin_amdgcn_readfirstlane(v); }
+
+// Lane l's value v, for a wave-uniform l (v_readlane: no LDS round trip).
+__device__ __forceinline__ int lane_value(int v, int l) { return __builtin_amdgcn_readlane(v, l); }
+
+// Minimum / maximum over the 64 lanes of a wave (wave-uniform result).
+__device__ __forceinline__ int wave_min(int v)
+{
+#pragma unroll
+    for (int d = 32; d >= 1; d >>= 1) v = min(v, __shfl_xor(v, d));
+    return uniform_int(v);
+}
+
+__device__ __forceinline__ int wave_max(int v)
+{
+#pragma unroll
+    for (int d = 32; d >= 1; d >>= 1) v = max(v, __shfl_xor(v, d));
+    return uniform_int(v);
+}
+
+// Inclusive prefix sum over the 64 lanes of a wave, in registers: Hillis-Steele
+// shifts by 1, 2, 4, 8 within each row of 16 lanes, then row 0's total into
+// row 1 and row 2's into row 3 (row_bcast:15), then rows 0-1's total into rows
+// 2-3 (row_bcast:31).  Lanes without a source add 0.
+__device__ __forceinline__ int wave_inclusive_scan(int v)
+{
+    v += __builtin_amdgcn_update_dpp(0, v, 0x111, 0xf, 0xf, false);  // row_shr:1
+    v += __builtin_amdgcn_update_dpp(0, v, 0x112, 0xf, 0xf, false);  // row_shr:2
+    v += __builtin_amdgcn_update_dpp(0, v, 0x114, 0xf, 0xf, false);  // row_shr:4
+    v += __builtin_amdgcn_update_dpp(0, v, 0x118, 0xf, 0xf, false);  // row_shr:8
+    v += __builtin_amdgcn_update_dpp(0, v, 0x142, 0xa, 0xf, false);  // row_bcast:15, rows 1 and 3
+    v += __builtin_amdgcn_update_dpp(0, v, 0x143, 0xc, 0xf, false);  // row_bcast:31, rows 2 and 3
+    return v;
+}
 
 // Raw buffer resource over n floats at p (CDNA: word 3 = 0x00020000, 32-bit
 // data format).  Loads through it are range-checked by the hardware: a byte

@@ -169,12 +169,12 @@ __global__ __launch_bounds__(256, 2) void fwd_strip_kernel(FwdArgs p)
         int gi = 0;
         while (gi < na) {
             // Longest run of angles gi..gj-1 whose union footprint fits the budget.
-            int rlo = __shfl(mine.rlo, gi), rhi = __shfl(mine.rhi, gi);
-            int clo = __shfl(mine.clo, gi), chi = __shfl(mine.chi, gi);
+            int rlo = lane_value(mine.rlo, gi), rhi = lane_value(mine.rhi, gi);
+            int clo = lane_value(mine.clo, gi), chi = lane_value(mine.chi, gi);
             int gj = gi + 1;
             while (gj < na) {
-                const int mrlo = min(rlo, __shfl(mine.rlo, gj)), mrhi = max(rhi, __shfl(mine.rhi, gj));
-                const int mclo = min(clo, __shfl(mine.clo, gj)), mchi = max(chi, __shfl(mine.chi, gj));
+                const int mrlo = min(rlo, lane_value(mine.rlo, gj)), mrhi = max(rhi, lane_value(mine.rhi, gj));
+                const int mclo = min(clo, lane_value(mine.clo, gj)), mchi = max(chi, lane_value(mine.chi, gj));
                 if ((mrhi - mrlo + 1) * ((mchi - mclo + 1) | 1) * NR > kFLds) break;
                 rlo = mrlo;
                 rhi = mrhi;
@@ -295,19 +295,22 @@ constexpr int kBWaves = 4;            // private accumulators per workgroup
 constexpr int kBPitch = kBJ + 2;      // 65: base-cell cols plus the right neighbour (odd)
 constexpr int kBAcc = (kBI + 2) * kBPitch;
 constexpr int kBRefChunk = 2;
-constexpr int kBSink = 136;           // per-wave sink cells: lane + corner offset <= 63 + 66
+constexpr int kBSink = 136;           // sink cells: lane + corner offset <= 63 + 66
 constexpr int kBItems = 32;           // items of 128 samples per enumeration window
 constexpr int kBWin = 128 * kBItems;  // samples per window
 constexpr int kBUnroll = 2;           // items splatted together
-constexpr int kBSlots = 256;          // election table for clamped samples (hash of the cell)
+constexpr int kBSlots = 64;           // election table for clamped samples (hash of the cell)
+constexpr int kBRows = 48;            // detector rows per batch
 static_assert(kBItems * 2 == 64, "one mask word per lane");
 static_assert(kBItems % kBUnroll == 0, "items come in whole unroll groups");
 
 // Dynamic LDS of the back-projector: four private accumulators (cell-major,
-// nref values per cell), then the sinks.
+// nref values per cell), then one sink shared by the waves (it only ever holds
+// discarded values, so their unordered writes there are harmless).  With the
+// static tables this stays under 40 KB for nref = 1: four workgroups per CU.
 constexpr size_t bwd_lds_bytes(int nref)
 {
-    return sizeof(float) * (size_t)kBWaves * nref * (kBAcc + kBSink);
+    return sizeof(float) * (size_t)nref * (kBWaves * kBAcc + kBSink);
 }
 
 enum BwdMode { kBwdWrite = 0, kBwdSens = 1, kBwdUpdate = 2, kBwdAccumulate = 3 };
@@ -385,8 +388,8 @@ struct BwdTile {
 };
 
 // Sample stream of one wave.  Angles a = wave, wave+4, ...; for each angle the
-// detector rows of the tile's preimage in two classes, h = hlo + cls + 2k, 64
-// rows per batch (lane k holds row k).  Each row's in-region samples form one
+// detector rows of the tile's preimage in two classes, h = hlo + cls + 2k,
+// kBRows rows per batch (lane k holds row k).  Each row's in-region samples form one
 // column interval, padded to an even count; a batch's samples, row after row,
 // are numbered f = 0..T-1, cut into windows of kBWin and items of 128: lane l
 // of item n holds the pair f = 128n+2l, 128n+2l+1 (points 0 and 1), always of
@@ -457,7 +460,7 @@ __device__ __forceinline__ void bwd_begin_batch(const BwdArgs &p, const BwdTile 
     const float c = cur.c, s = cur.s;
     const int h = cur.rb + 2 * lane;
     int ql = 0, len = 0;
-    if (h <= cur.hhi) {
+    if (lane < kBRows && h <= cur.hhi) {
         const float hc = (float)h - cy;
         const float bx = fmaf(c, hc, cy), by = fmaf(-s, hc, cx);
         auto in_region = [&](int q) {
@@ -479,13 +482,8 @@ __device__ __forceinline__ void bwd_begin_batch(const BwdArgs &p, const BwdTile 
         len = max(qb - qa + 1, 0);
     }
     const int lp = len + (len & 1);
-    int P = lp;  // inclusive prefix sum over the batch
-#pragma unroll
-    for (int d = 1; d < 64; d <<= 1) {
-        const int v = __shfl_up(P, d);
-        if (lane >= d) P += v;
-    }
-    cur.T = uniform_int(__shfl(P, 63));
+    int P = wave_inclusive_scan(lp);  // inclusive prefix sum over the batch
+    cur.T = lane_value(P, 63);
     P -= lp;
     const bool ne = len > 0;
     const uint64_t nem = __ballot(ne);
@@ -524,7 +522,7 @@ __device__ __forceinline__ bool bwd_next(const BwdArgs &p, const BwdTile &t, Bwd
                 bwd_begin_window(t, cur);
                 return true;
             }
-            cur.rb += 128;
+            cur.rb += 2 * kBRows;
             cur.stage = 1;
         } else if (cur.stage == 1) {
             if (cur.rb > cur.hhi) {
@@ -539,7 +537,7 @@ __device__ __forceinline__ bool bwd_next(const BwdArgs &p, const BwdTile &t, Bwd
             }
             bwd_begin_batch(p, t, cur);
             if (cur.T == 0) {
-                cur.rb += 128;
+                cur.rb += 2 * kBRows;
                 continue;
             }
             cur.win0 = 0;
@@ -731,7 +729,7 @@ __global__ __launch_bounds__(256) void bwd_tile_kernel(BwdArgs p)
     extern __shared__ float lds[];
     __shared__ uint32_t s_owner[kBWaves][kBSlots];
     __shared__ __align__(16) uint32_t s_mask[kBWaves][kBItems * 2];
-    __shared__ int4 s_row[kBWaves][64];
+    __shared__ int4 s_row[kBWaves][kBRows];
 
     uint32_t id = xcd_remap(blockIdx.x, gridDim.x);
     const int tj = id % p.ntj;
@@ -764,7 +762,7 @@ __global__ __launch_bounds__(256) void bwd_tile_kernel(BwdArgs p)
     t.lane = lane;
     // Sink words (after all accumulators): lanes with nothing to add in a step
     // read-modify-write there, each lane its own NR words.
-    t.sink = kBWaves * kBAcc + wave * kBSink + lane;
+    t.sink = kBWaves * kBAcc + lane;
     for (int k = lane; k < nacc; k += 64) lds[wave * nacc + k] = 0.0f;
     for (int k = lane; k < kBSlots; k += 64) t.owner[k] = 0xffffffffu;
     const bool border = t.lox < 0.0f || t.hix > (float)p.H || t.loy < 0.0f || t.hiy > (float)p.W;
