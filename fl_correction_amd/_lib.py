@@ -54,7 +54,20 @@ SIGNATURES = {
     "fl_incident_grad": (_INT, [_P, _P, _P, _P, _P, _I64, _I64, _I64, _I64, _P]),
     "fl_mu_adjoint": (_INT, [_P, _P, _P, _P, _P, _I64, _I64, _I64, _I64, _I64, _INT, _P]),
     "fl_rotate_adjoint": (_INT, [_P, _P, _I64, _I64, _I64, _F, _F, _INT, _P]),
+    "fl_mlem_multi_workspace_bytes": (_SZ, [_P, _I64]),
+    "fl_mlem_multi_create": (_INT, [_P, _I64, _D, _D, _P, _SZ, _P, ctypes.POINTER(_P)]),
+    "fl_mlem_multi_run": (_INT, [_P, _I64, _INT, _P]),
+    "fl_mlem_multi_destroy": (_INT, [_P]),
 }
+
+
+class MLEMProblem(ctypes.Structure):
+    """fl_mlem_problem of include/fl_correction_amd.h."""
+    _fields_ = [
+        ("atten", _P), ("em_cs", _P), ("theta", _P), ("I", _P), ("C", _P),
+        ("A", _I64), ("B", _I64), ("H", _I64), ("W", _I64),
+        ("fwd_angles", ctypes.c_int32), ("bwd_rows", ctypes.c_int32),
+    ]
 
 
 class FLError(RuntimeError):

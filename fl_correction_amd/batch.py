@@ -1,0 +1,224 @@
+"""Batched MLEM over many independent problems of uneven size.
+
+A problem is one emission line of one sample (the reference's
+``torch_mlem_recon_batch`` with one reference, FL/torch_recon.py:50-98):
+atten (A,B,H,W), em_cs (A,), theta (A,) in radians, measured I (A,B,W) and
+the estimate C (B,H,W), which is updated in place.  The reference
+reconstructs such problems one call at a time; ``BatchMLEM`` hands all of
+them to the library's ``fl_mlem_multi_*`` entry points, which run every pass
+of an iteration as one persistent launch over all problems and replay a
+captured hipGraph of two iterations.
+
+Tile shapes are chosen per problem by ``Autotuner``: for each size bucket
+(H and W rounded up to multiples of 32) it times full iterations of a
+synthetic batch of that bucket under every combination of forward angles per
+workgroup (8, 4) and back-projector tile height (31, 15), keeps the fastest,
+and caches the choice in a JSON file keyed by device name and bucket.
+"""
+from __future__ import annotations
+
+import ctypes
+import json
+import os
+from dataclasses import dataclass
+
+import torch
+
+from . import _lib
+
+FWD_ANGLES = (8, 4)
+BWD_ROWS = (31, 15)
+
+
+@dataclass
+class Problem:
+    atten: torch.Tensor  # (A,B,H,W)
+    em_cs: torch.Tensor  # (A,)
+    theta: torch.Tensor  # (A,) radians
+    I: torch.Tensor      # (A,B,W)
+    C: torch.Tensor      # (B,H,W), updated in place
+
+    def dims(self):
+        A, B, H, W = (int(x) for x in self.atten.shape)
+        return A, B, H, W
+
+
+def _check(p: Problem, i: int):
+    A, B, H, W = p.dims()
+    want = {"atten": (A, B, H, W), "em_cs": (A,), "theta": (A,), "I": (A, B, W), "C": (B, H, W)}
+    dev = p.atten.device
+    for name, shape in want.items():
+        t = getattr(p, name)
+        if not isinstance(t, torch.Tensor) or tuple(t.shape) != shape:
+            raise ValueError(f"problem {i}: {name} must have shape {shape}")
+        if not (t.is_cuda and t.dtype == torch.float32 and t.is_contiguous() and t.device == dev):
+            raise ValueError(f"problem {i}: {name} must be a contiguous float32 tensor on {dev}")
+
+
+def _structs(problems, shapes):
+    arr = (_lib.MLEMProblem * len(problems))()
+    for i, (p, (fa, br)) in enumerate(zip(problems, shapes)):
+        A, B, H, W = p.dims()
+        arr[i] = _lib.MLEMProblem(p.atten.data_ptr(), p.em_cs.data_ptr(), p.theta.data_ptr(), p.I.data_ptr(),
+                                  p.C.data_ptr(), A, B, H, W, int(fa), int(br))
+    return arr
+
+
+def workspace_bytes(problems, shapes=None):
+    """Bytes of device workspace fl_mlem_multi_create needs (host arithmetic only)."""
+    shapes = shapes or [(0, 0)] * len(problems)
+    arr = _structs(problems, shapes) if problems and isinstance(problems[0], Problem) else problems
+    return int(_lib.load().fl_mlem_multi_workspace_bytes(ctypes.cast(arr, ctypes.c_void_p), len(arr)))
+
+
+class BatchMLEM:
+    """All problems of a batch, iterated together.
+
+    shapes: None (library defaults), a list of (fwd_angles, bwd_rows) per
+    problem, or an ``Autotuner``.  beta > 0 selects the One-Step-Late Huber
+    update (atten_cuda.cpp:330-341), as in ``mlem_cuda_batch``.
+    """
+
+    def __init__(self, problems, beta=0.0, delta=0.01, shapes=None):
+        if not problems:
+            raise ValueError("empty batch")
+        for i, p in enumerate(problems):
+            _check(p, i)
+        self.device = problems[0].atten.device
+        if any(p.atten.device != self.device for p in problems):
+            raise ValueError("all problems must be on one device")
+        if isinstance(shapes, Autotuner):
+            shapes = [shapes.shapes(*p.dims(), device=self.device) for p in problems]
+        self.shapes = list(shapes) if shapes is not None else [(0, 0)] * len(problems)
+        if len(self.shapes) != len(problems):
+            raise ValueError("one tile shape per problem")
+        self.problems = list(problems)  # keeps every array alive while the plan uses it
+        self._arr = _structs(self.problems, self.shapes)
+        lib = _lib.load()
+        nbytes = int(lib.fl_mlem_multi_workspace_bytes(ctypes.cast(self._arr, ctypes.c_void_p), len(problems)))
+        if nbytes == 0:
+            raise ValueError("invalid batch")
+        # 256-byte aligned workspace inside a slightly larger buffer
+        self._ws_buf = torch.empty(nbytes + 256, dtype=torch.uint8, device=self.device)
+        off = (-self._ws_buf.data_ptr()) % 256
+        self._ws = self._ws_buf.data_ptr() + off
+        self._nbytes = nbytes
+        # graph capture needs a stream other than the NULL stream
+        self.stream = torch.cuda.Stream(self.device)
+        self._plan = ctypes.c_void_p()
+        cur = torch.cuda.current_stream(self.device)
+        self.stream.wait_stream(cur)
+        with torch.cuda.device(self.device):
+            _lib.call("fl_mlem_multi_create", ctypes.cast(self._arr, ctypes.c_void_p), len(problems),
+                      float(beta), float(delta), ctypes.c_void_p(self._ws), nbytes,
+                      ctypes.c_void_p(self.stream.cuda_stream), ctypes.byref(self._plan))
+        cur.wait_stream(self.stream)
+
+    def run(self, n_iter, graph=True):
+        """n_iter iterations of every problem; C of each problem holds the result."""
+        if not self._plan:
+            raise RuntimeError("plan released")
+        cur = torch.cuda.current_stream(self.device)
+        self.stream.wait_stream(cur)
+        with torch.cuda.device(self.device):
+            _lib.call("fl_mlem_multi_run", self._plan, int(n_iter), 1 if graph else 0,
+                      ctypes.c_void_p(self.stream.cuda_stream))
+        cur.wait_stream(self.stream)
+
+    def close(self):
+        if self._plan:
+            self.stream.synchronize()
+            _lib.call("fl_mlem_multi_destroy", self._plan)
+            self._plan = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def _default_cache():
+    base = os.environ.get("XDG_CACHE_HOME") or os.path.join(os.path.expanduser("~"), ".cache")
+    return os.path.join(base, "fl_correction_amd", "autotune.json")
+
+
+class Autotuner:
+    """Per size bucket tile shapes for the batched MLEM kernels.
+
+    The bucket of a problem is (H, W) rounded up to multiples of 32.  A bucket
+    is tuned once per device: a synthetic batch of that bucket (enough slices
+    to fill the GPU several times over) runs ``reps`` timed iterations under
+    each (fwd_angles, bwd_rows) pair; the fastest is cached in ``path``.
+    """
+
+    def __init__(self, path=None, reps=3, tune_angles=120):
+        self.path = path or os.environ.get("FL_AUTOTUNE_CACHE") or _default_cache()
+        self.reps = int(reps)
+        self.tune_angles = int(tune_angles)
+        self.table = {}
+        try:
+            with open(self.path) as f:
+                self.table = json.load(f)
+        except (OSError, ValueError):
+            self.table = {}
+        self.timings = {}
+
+    @staticmethod
+    def bucket(H, W):
+        return (-(-int(H) // 32) * 32, -(-int(W) // 32) * 32)
+
+    @staticmethod
+    def key(device_name, H, W):
+        bh, bw = Autotuner.bucket(H, W)
+        return f"{device_name}|{bh}x{bw}"
+
+    def shapes(self, A, B, H, W, device=None):
+        device = torch.device(device or "cuda")
+        k = self.key(torch.cuda.get_device_name(device), H, W)
+        if k not in self.table:
+            self.table[k] = list(self._tune(*self.bucket(H, W), device))
+            self._save()
+        fa, br = self.table[k]
+        return int(fa), int(br)
+
+    def _save(self):
+        try:
+            os.makedirs(os.path.dirname(self.path), exist_ok=True)
+            tmp = self.path + ".tmp"
+            with open(tmp, "w") as f:
+                json.dump(self.table, f, indent=1, sort_keys=True)
+            os.replace(tmp, self.path)
+        except OSError:
+            pass  # the choice is still used for this process
+
+    def _tune(self, H, W, device):
+        A = self.tune_angles
+        # slices so that the back-projector has >= 4 tiles per resident workgroup
+        props = torch.cuda.get_device_properties(device)
+        tiles_per_slice = -(-H // 31) * -(-W // 63)
+        B = max(4, min(64, -(-4 * 4 * props.multi_processor_count // tiles_per_slice)))
+        g = torch.Generator(device=device).manual_seed(1234)
+        atten = torch.rand((A, B, H, W), generator=g, device=device) * 0.9 + 0.05
+        em = torch.ones(A, device=device)
+        theta = torch.arange(A, device=device, dtype=torch.float32) * (3.14159265 / A)
+        I = torch.rand((A, B, W), generator=g, device=device) + 0.5
+        C0 = torch.ones((B, H, W), device=device)
+        best, best_ms = None, float("inf")
+        for fa in FWD_ANGLES:
+            for br in BWD_ROWS:
+                prob = Problem(atten, em, theta, I, C0.clone())
+                run = BatchMLEM([prob], shapes=[(fa, br)])
+                run.run(1, graph=False)
+                start = torch.cuda.Event(enable_timing=True)
+                stop = torch.cuda.Event(enable_timing=True)
+                start.record(run.stream)
+                run.run(2 * self.reps, graph=True)
+                stop.record(run.stream)
+                stop.synchronize()
+                ms = start.elapsed_time(stop) / (2 * self.reps)
+                run.close()
+                self.timings[(H, W, fa, br)] = ms
+                if ms < best_ms:
+                    best, best_ms = (fa, br), ms
+        return best

@@ -112,6 +112,40 @@ __device__ __forceinline__ float scalar_read(const float *p, int64_t i)
     return ((__attribute__((address_space(4))) const float *)p)[i];
 }
 
+__device__ __forceinline__ uint32_t scalar_read_u32(const uint32_t *p, int64_t i)
+{
+    return ((__attribute__((address_space(4))) const uint32_t *)p)[i];
+}
+
+// A wave-uniform record (kernel argument block, job table entry) through the
+// scalar data cache.
+template <class T>
+__device__ __forceinline__ T scalar_record(const T *p)
+{
+    static_assert(sizeof(T) % 4 == 0, "records are read in 32-bit words");
+    T r;
+    uint32_t *d = reinterpret_cast<uint32_t *>(&r);
+    const auto *s = (__attribute__((address_space(4))) const uint32_t *)p;
+#pragma unroll
+    for (size_t i = 0; i < sizeof(T) / 4; ++i) d[i] = s[i];
+    return r;
+}
+
+// Job of logical tile t in a grouped launch: the last j with begin[j] <= t
+// (begin ascending, begin[0] = 0).  Wave-uniform.
+__device__ __forceinline__ int find_job(const uint32_t *begin, int n_jobs, uint32_t t)
+{
+    int lo = 0, hi = n_jobs - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (scalar_read_u32(begin, mid) <= t)
+            lo = mid;
+        else
+            hi = mid - 1;
+    }
+    return lo;
+}
+
 // One float at byte offset off (0 past the end).
 __device__ __forceinline__ float load_f32(__amdgpu_buffer_rsrc_t r, uint32_t off)
 {

@@ -28,7 +28,7 @@
 // bilinear weights of the points that land in the tile are splatted into the
 // wave's own accumulator.  No global atomics; the four accumulators are summed in
 // a fixed order at the end, which is also where the MLEM update is applied.
-#include "common.hpp"
+#include "projector.hpp"
 
 #include <algorithm>
 #include <climits>
@@ -38,32 +38,6 @@ namespace fl {
 // ---------------------------------------------------------------------------
 // Forward projector
 // ---------------------------------------------------------------------------
-constexpr int kFW = 64;                   // detector columns per workgroup (one per lane)
-constexpr int kFH = 32;                   // detector rows per chunk
-constexpr int kFWaves = 4;
-constexpr int kFRows = kFH / kFWaves;     // rows per wave per chunk
-constexpr int kFG = 8;                    // angles per workgroup (one accumulator each)
-constexpr int kFLds = 11264;              // footprint floats staged per workgroup (44 KB)
-// One angle's footprint for a 64x32 tile: the sample points span at most
-// hypot(31, 63) < 70.3 rows (and cols); with the 0.5 margins and the bilinear
-// neighbour that is <= 74 rows and a pitch <= 75.
-constexpr int kFMaxFootprint = 74 * 75;
-constexpr int kFRefChunk = kFLds / kFMaxFootprint;  // references staged together (2)
-
-enum FwdMode { kFwdWrite = 0, kFwdAccumulate = 1, kFwdRatio = 2, kFwdAccumulateRatio = 3 };
-
-struct FwdArgs {
-    const float *atten;
-    const float *C;
-    const float *em;
-    const float *theta;
-    const float *I;  // ratio modes
-    float *out;      // (A, B, W): Pf or ratio
-    int A, B, R, H, W;
-    int r0, nref;
-    int mode;
-    int ntw, n_groups;
-};
 
 struct Box {
     int rlo, rhi, clo, chi;
@@ -96,14 +70,15 @@ __device__ __forceinline__ Box fwd_footprint(float c, float s, int h0, int h1, i
     return b;
 }
 
-template <int NR>
-__global__ __launch_bounds__(256, 2) void fwd_strip_kernel(FwdArgs p)
+// One workgroup's tile t: detector strip t % ntw of angle group (t / ntw) %
+// n_groups of slice t / (ntw * n_groups); G angles per group.
+template <int NR, int G>
+__device__ __forceinline__ void fwd_strip_tile(const FwdArgs &p, uint32_t t)
 {
     extern __shared__ float lds[];
-    __shared__ float trig[2 * kFG];
-    __shared__ float emv[kFG * kFRefChunk];
+    __shared__ float trig[2 * G];
+    __shared__ float emv[G * kFRefChunk];
 
-    uint32_t t = xcd_remap(blockIdx.x, gridDim.x);
     const int tw = t % p.ntw;
     t /= p.ntw;
     const int g = t % p.n_groups;
@@ -117,27 +92,27 @@ __global__ __launch_bounds__(256, 2) void fwd_strip_kernel(FwdArgs p)
     const float cx = (p.W - 1) * 0.5f, cy = (p.H - 1) * 0.5f;
     const float xmax = p.H - 1.001f, ymax = p.W - 1.001f;
     const int64_t HW = (int64_t)p.H * p.W;
-    const int a0 = g * kFG;
-    const int na = min(kFG, p.A - a0);
+    const int a0 = g * G;
+    const int na = min(G, p.A - a0);
     // Lanes past the last column sample the last valid column (their atten is 0),
     // so every LDS index stays inside the staged footprint.
     const float wc = (float)min(w, w1 - 1) - cx;
 
-    if (threadIdx.x < kFG) {
+    if (threadIdx.x < G) {
         const int k = threadIdx.x;
         const float th = k < na ? finite_or_zero(p.theta[a0 + k]) : 0.0f;
         trig[2 * k] = cosf(-th);
         trig[2 * k + 1] = sinf(-th);
     }
-    if (threadIdx.x < kFG * kFRefChunk) {
+    if (threadIdx.x < G * kFRefChunk) {
         const int k = threadIdx.x / kFRefChunk, rr = threadIdx.x % kFRefChunk;
         emv[threadIdx.x] = (k < na && rr < NR) ? p.em[(int64_t)(a0 + k) * p.R + p.r0 + rr] : 0.0f;
     }
     __syncthreads();
 
-    float acc[kFG];
+    float acc[G];
 #pragma unroll
-    for (int k = 0; k < kFG; ++k) acc[k] = 0.0f;
+    for (int k = 0; k < G; ++k) acc[k] = 0.0f;
 
     for (int h0 = 0; h0 < p.H; h0 += kFH) {
         const int h1 = min(h0 + kFH, p.H);
@@ -147,10 +122,10 @@ __global__ __launch_bounds__(256, 2) void fwd_strip_kernel(FwdArgs p)
         // Bounds-checked buffer loads: rows past the image and lanes past the
         // last column (offset moved out of range) read 0; the row offset is a
         // scalar operand, so no vector address arithmetic per load.
-        float av[kFG][kFRows];
+        float av[G][kFRows];
         const uint32_t voff = wvalid ? (uint32_t)(hr0 * p.W + w) * 4u : 0x80000000u;
 #pragma unroll
-        for (int k = 0; k < kFG; ++k) {
+        for (int k = 0; k < G; ++k) {
             const __amdgpu_buffer_rsrc_t ra =
                 make_rsrc(p.atten + ((int64_t)(a0 + min(k, na - 1)) * p.B + b) * HW, k < na ? (int)HW : 0);
 #pragma unroll
@@ -163,7 +138,7 @@ __global__ __launch_bounds__(256, 2) void fwd_strip_kernel(FwdArgs p)
         // Footprint of every angle of the group, lane k computing angle k's box.
         Box mine{0, 0, 0, 0};
         {
-            const int k = min(lane, kFG - 1);
+            const int k = min(lane, G - 1);
             mine = fwd_footprint(trig[2 * k], trig[2 * k + 1], h0, h1, w0, w1, cx, cy, xmax, ymax);
         }
         int gi = 0;
@@ -221,7 +196,7 @@ __global__ __launch_bounds__(256, 2) void fwd_strip_kernel(FwdArgs p)
             const char *fb = reinterpret_cast<const char *>(lds) - 4 * (rlo * pitch + clo);
             const uint32_t pitch4 = 4u * (uint32_t)pitch;
 #pragma unroll
-            for (int k = 0; k < kFG; ++k) {
+            for (int k = 0; k < G; ++k) {
                 if (k < gi || k >= gj) continue;  // wave-uniform
                 // sample position of this lane's column at row h: (c*(h-cy) + bx, s*(h-cy) + by)
                 const float c = trig[2 * k], s = trig[2 * k + 1];
@@ -264,9 +239,9 @@ __global__ __launch_bounds__(256, 2) void fwd_strip_kernel(FwdArgs p)
 
     // Fixed-order reduction of the four waves' partial sums.
     __syncthreads();
-    float *red = lds;  // [kFG][kFWaves][64]
+    float *red = lds;  // [G][kFWaves][64]
 #pragma unroll
-    for (int k = 0; k < kFG; ++k) red[(k * kFWaves + wave) * 64 + lane] = acc[k];
+    for (int k = 0; k < G; ++k) red[(k * kFWaves + wave) * 64 + lane] = acc[k];
     __syncthreads();
     for (int idx = threadIdx.x; idx < na * 64; idx += 256) {
         const int k = idx >> 6, l = idx & 63;
@@ -286,49 +261,15 @@ __global__ __launch_bounds__(256, 2) void fwd_strip_kernel(FwdArgs p)
     }
 }
 
+template <int NR, int G>
+__global__ __launch_bounds__(256, 2) void fwd_strip_kernel(FwdArgs p)
+{
+    fwd_strip_tile<NR, G>(p, xcd_remap(blockIdx.x, gridDim.x));
+}
+
 // ---------------------------------------------------------------------------
 // Back-projection (exact transpose) and the fused MLEM update
 // ---------------------------------------------------------------------------
-constexpr int kBI = 31;               // image rows per tile (base cells of its samples: 32 rows)
-constexpr int kBJ = 63;               // image cols per tile (base cells: 64 cols)
-constexpr int kBWaves = 4;            // private accumulators per workgroup
-constexpr int kBPitch = kBJ + 2;      // 65: base-cell cols plus the right neighbour (odd)
-constexpr int kBAcc = (kBI + 2) * kBPitch;
-constexpr int kBRefChunk = 2;
-constexpr int kBSink = 136;           // sink cells: lane + corner offset <= 63 + 66
-constexpr int kBItems = 32;           // items of 128 samples per enumeration window
-constexpr int kBWin = 128 * kBItems;  // samples per window
-constexpr int kBUnroll = 2;           // items splatted together
-constexpr int kBSlots = 64;           // election table for clamped samples (hash of the cell)
-constexpr int kBRows = 48;            // detector rows per batch
-static_assert(kBItems * 2 == 64, "one mask word per lane");
-static_assert(kBItems % kBUnroll == 0, "items come in whole unroll groups");
-
-// Dynamic LDS of the back-projector: four private accumulators (cell-major,
-// nref values per cell), then one sink shared by the waves (it only ever holds
-// discarded values, so their unordered writes there are harmless).  With the
-// static tables this stays under 40 KB for nref = 1: four workgroups per CU.
-constexpr size_t bwd_lds_bytes(int nref)
-{
-    return sizeof(float) * (size_t)nref * (kBWaves * kBAcc + kBSink);
-}
-
-enum BwdMode { kBwdWrite = 0, kBwdSens = 1, kBwdUpdate = 2, kBwdAccumulate = 3 };
-
-struct BwdArgs {
-    const float *atten;
-    const float *ratio;  // (A,B,W) or null (= 1)
-    const float *em;
-    const float *theta;
-    float *out;  // (R,B,H,W)
-    int A, B, R, H, W;
-    int r0, nref;
-    int nti, ntj;
-    int mode;
-    const float *C;     // update: current estimate (R,B,H,W)
-    const float *sens;  // update: sensitivity (R,B,H,W)
-    float beta, delta;
-};
 
 __device__ __forceinline__ float huber_at(const float *Cs, int i, int j, int H, int W, float delta)
 {
@@ -475,7 +416,7 @@ __device__ __forceinline__ void bwd_begin_batch(const BwdArgs &p, const BwdTile 
         int qb = (int)clampf(ceilf(qmax + cx), -1.f, (float)(p.W - 1));
         // Tighten to the exact set: with the same arithmetic as the splat, a
         // sample passes this test iff its clamped base cell lies in
-        // [I0-1, I0+kBI-1] x [J0-1, J0+kBJ-1].
+        // [I0-1, I0+BI-1] x [J0-1, J0+kBJ-1] (BI the tile height).
         while (qa <= qb && !in_region(qa)) ++qa;
         while (qa <= qb && !in_region(qb)) --qb;
         ql = qa;
@@ -723,46 +664,60 @@ __device__ __forceinline__ void bwd_stream(const BwdArgs &p, const BwdTile &t, i
     }
 }
 
-template <int NR>
-__global__ __launch_bounds__(256) void bwd_tile_kernel(BwdArgs p)
-{
-    extern __shared__ float lds[];
-    __shared__ uint32_t s_owner[kBWaves][kBSlots];
-    __shared__ __align__(16) uint32_t s_mask[kBWaves][kBItems * 2];
-    __shared__ int4 s_row[kBWaves][kBRows];
+// Per-wave tables of the back-projector in static LDS.  One instance per
+// kernel: every tile height a kernel instantiates shares it.
+struct BwdTables {
+    uint32_t owner[kBWaves][kBSlots];                  // election table (clamped samples)
+    __align__(16) uint32_t mask[kBWaves][kBItems * 2];  // row starts of the current window
+    int4 row[kBWaves][kBRows];                         // compacted rows of the current batch
+};
 
-    uint32_t id = xcd_remap(blockIdx.x, gridDim.x);
+__device__ __forceinline__ BwdTables &bwd_tables()
+{
+    __shared__ BwdTables tab;
+    return tab;
+}
+
+// One workgroup's tile id: image tile (id / ntj) % nti, id % ntj of slice
+// id / (nti * ntj), BI image rows high.
+template <int NR, int BI>
+__device__ __forceinline__ void bwd_tile(const BwdArgs &p, uint32_t id)
+{
+    constexpr int BACC = bwd_acc_cells(BI);
+    extern __shared__ float lds[];
+    BwdTables &tab = bwd_tables();
+
     const int tj = id % p.ntj;
     id /= p.ntj;
     const int ti = id % p.nti;
     const int b = id / p.nti;
     const int lane = threadIdx.x & 63;
     const int wave = uniform_int(threadIdx.x >> 6);  // wave-uniform: keeps per-wave state in SGPRs
-    constexpr int nacc = NR * kBAcc;
+    constexpr int nacc = NR * BACC;
 
     BwdTile t;
-    t.I0 = ti * kBI;
+    t.I0 = ti * BI;
     t.J0 = tj * kBJ;
     t.b = b;
     constexpr float BIG = 1.0e6f;
     // Unclamped sample positions whose clamped base cell lies in
-    // [I0-1, I0+kBI-1] x [J0-1, J0+kBJ-1]: open on a side where the clamp folds
+    // [I0-1, I0+BI-1] x [J0-1, J0+kBJ-1]: open on a side where the clamp folds
     // everything beyond onto the tile.
     t.lox = t.I0 <= 1 ? -BIG : (float)(t.I0 - 1);
-    t.hix = t.I0 + kBI >= p.H - 1 ? BIG : (float)(t.I0 + kBI);
+    t.hix = t.I0 + BI >= p.H - 1 ? BIG : (float)(t.I0 + BI);
     t.loy = t.J0 <= 1 ? -BIG : (float)(t.J0 - 1);
     t.hiy = t.J0 + kBJ >= p.W - 1 ? BIG : (float)(t.J0 + kBJ);
     // Largest rounding error of one sample coordinate near an active bound.
     t.tol = 2.4e-7f * (float)(2 * max(p.H, p.W) + 4);
     t.lds = lds;
-    t.cbase = wave * kBAcc + (1 - t.I0) * kBPitch + 1 - t.J0;
-    t.owner = s_owner[wave];
-    t.mask = s_mask[wave];
-    t.rows = s_row[wave];
+    t.cbase = wave * BACC + (1 - t.I0) * kBPitch + 1 - t.J0;
+    t.owner = tab.owner[wave];
+    t.mask = tab.mask[wave];
+    t.rows = tab.row[wave];
     t.lane = lane;
     // Sink words (after all accumulators): lanes with nothing to add in a step
     // read-modify-write there, each lane its own NR words.
-    t.sink = kBWaves * kBAcc + lane;
+    t.sink = kBWaves * BACC + lane;
     for (int k = lane; k < nacc; k += 64) lds[wave * nacc + k] = 0.0f;
     for (int k = lane; k < kBSlots; k += 64) t.owner[k] = 0xffffffffu;
     const bool border = t.lox < 0.0f || t.hix > (float)p.H || t.loy < 0.0f || t.hiy > (float)p.W;
@@ -781,7 +736,7 @@ __global__ __launch_bounds__(256) void bwd_tile_kernel(BwdArgs p)
     for (int rr = 0; rr < NR; ++rr) {
         const int r = p.r0 + rr;
         const int64_t sb = ((int64_t)r * p.B + b) * HW;
-        for (int ii = wave; ii < kBI; ii += kBWaves) {
+        for (int ii = wave; ii < BI; ii += kBWaves) {
             const int i = I0 + ii;
             if (i >= p.H || j >= p.W || lane >= kBJ) continue;
             const int li = ((ii + 1) * kBPitch + lane + 1) * NR + rr;
@@ -802,6 +757,66 @@ __global__ __launch_bounds__(256) void bwd_tile_kernel(BwdArgs p)
                 p.out[o] = v < 0.0f ? 0.0f : v;
             }
         }
+    }
+}
+
+template <int NR, int BI>
+__global__ __launch_bounds__(256) void bwd_tile_kernel(BwdArgs p)
+{
+    bwd_tile<NR, BI>(p, xcd_remap(blockIdx.x, gridDim.x));
+}
+
+// ---------------------------------------------------------------------------
+// Grouped persistent launches.  The grid holds as many workgroups as are
+// resident at once; worker w (hardware id remapped so that each XCD holds one
+// contiguous range of workers) takes logical tiles w, w + grid, ...: in every
+// round each XCD works on one contiguous run of tiles, which share slices of
+// C and atten in that XCD's L2.  Jobs are ordered by cost per tile, largest
+// first, so that the rounds stay balanced.  A barrier separates consecutive
+// tiles of one workgroup (the next tile's LDS writes wait for the previous
+// tile's cross-wave reads).
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256, 2) void fwd_group_kernel(const FwdArgs *jobs, const uint32_t *begin, int n_jobs,
+                                                           uint32_t n_tiles)
+{
+    const uint32_t w = xcd_remap(blockIdx.x, gridDim.x);
+    for (uint32_t t = w; t < n_tiles; t += gridDim.x) {
+        const int j = find_job(begin, n_jobs, t);
+        const FwdArgs p = scalar_record(jobs + j);
+        const uint32_t lt = t - scalar_read_u32(begin, j);
+        if (t != w) __syncthreads();
+        if (p.group_angles == 4)
+            fwd_strip_tile<1, 4>(p, lt);
+        else
+            fwd_strip_tile<1, 8>(p, lt);
+    }
+}
+
+__global__ __launch_bounds__(256) void bwd_group_kernel(const BwdArgs *jobs, const uint32_t *begin, int n_jobs,
+                                                        uint32_t n_tiles)
+{
+    const uint32_t w = xcd_remap(blockIdx.x, gridDim.x);
+    for (uint32_t t = w; t < n_tiles; t += gridDim.x) {
+        const int j = find_job(begin, n_jobs, t);
+        const BwdArgs p = scalar_record(jobs + j);
+        const uint32_t lt = t - scalar_read_u32(begin, j);
+        if (t != w) __syncthreads();
+        if (p.tile_rows == 15)
+            bwd_tile<1, 15>(p, lt);
+        else
+            bwd_tile<1, 31>(p, lt);
+    }
+}
+
+__global__ __launch_bounds__(256) void copy_group_kernel(const CopyJob *jobs, const uint32_t *begin, int n_jobs,
+                                                         uint32_t n_tiles)
+{
+    for (uint32_t t = blockIdx.x; t < n_tiles; t += gridDim.x) {
+        const int j = find_job(begin, n_jobs, t);
+        const CopyJob c = scalar_record(jobs + j);
+        const int64_t i0 = (int64_t)(t - scalar_read_u32(begin, j)) * kCopyTile;
+        const int64_t i1 = i0 + kCopyTile < c.n ? i0 + kCopyTile : c.n;
+        for (int64_t i = i0 + threadIdx.x; i < i1; i += 256) c.dst[i] = c.src[i];
     }
 }
 
@@ -827,9 +842,9 @@ int launch_forward(const float *atten, const float *C, const float *em, const fl
     static bool attr_set = false;
     const size_t shmem = sizeof(float) * kFLds;
     if (!attr_set) {
-        FL_HIP_CHECK(hipFuncSetAttribute((const void *)fwd_strip_kernel<1>, hipFuncAttributeMaxDynamicSharedMemorySize,
+        FL_HIP_CHECK(hipFuncSetAttribute((const void *)fwd_strip_kernel<1, kFG>, hipFuncAttributeMaxDynamicSharedMemorySize,
                                          (int)shmem));
-        FL_HIP_CHECK(hipFuncSetAttribute((const void *)fwd_strip_kernel<2>, hipFuncAttributeMaxDynamicSharedMemorySize,
+        FL_HIP_CHECK(hipFuncSetAttribute((const void *)fwd_strip_kernel<2, kFG>, hipFuncAttributeMaxDynamicSharedMemorySize,
                                          (int)shmem));
         attr_set = true;
     }
@@ -857,9 +872,9 @@ int launch_forward(const float *atten, const float *C, const float *em, const fl
         p.mode = first ? (last && ratio ? kFwdRatio : kFwdWrite)
                        : (last && ratio ? kFwdAccumulateRatio : kFwdAccumulate);
         if (p.nref == 2)
-            hipLaunchKernelGGL(fwd_strip_kernel<2>, dim3((uint32_t)nwg), dim3(256), shmem, stream, p);
+            fwd_strip_kernel<2, kFG><<<dim3((uint32_t)nwg), dim3(256), shmem, stream>>>(p);
         else
-            hipLaunchKernelGGL(fwd_strip_kernel<1>, dim3((uint32_t)nwg), dim3(256), shmem, stream, p);
+            fwd_strip_kernel<1, kFG><<<dim3((uint32_t)nwg), dim3(256), shmem, stream>>>(p);
         FL_LAUNCH_CHECK();
     }
     return FL_OK;
@@ -871,9 +886,9 @@ int launch_backward(const float *atten, const float *ratio, const float *em, con
 {
     static bool attr_set = false;
     if (!attr_set) {
-        FL_HIP_CHECK(hipFuncSetAttribute((const void *)bwd_tile_kernel<1>, hipFuncAttributeMaxDynamicSharedMemorySize,
+        FL_HIP_CHECK(hipFuncSetAttribute((const void *)bwd_tile_kernel<1, kBI>, hipFuncAttributeMaxDynamicSharedMemorySize,
                                          (int)bwd_lds_bytes(1)));
-        FL_HIP_CHECK(hipFuncSetAttribute((const void *)bwd_tile_kernel<2>, hipFuncAttributeMaxDynamicSharedMemorySize,
+        FL_HIP_CHECK(hipFuncSetAttribute((const void *)bwd_tile_kernel<2, kBI>, hipFuncAttributeMaxDynamicSharedMemorySize,
                                          (int)bwd_lds_bytes(2)));
         attr_set = true;
     }
@@ -901,9 +916,9 @@ int launch_backward(const float *atten, const float *ratio, const float *em, con
         p.r0 = (int)r0;
         p.nref = (int)std::min<int64_t>(kBRefChunk, R - r0);
         if (p.nref == 2)
-            hipLaunchKernelGGL(bwd_tile_kernel<2>, dim3((uint32_t)nwg), dim3(256), bwd_lds_bytes(2), stream, p);
+            bwd_tile_kernel<2, kBI><<<dim3((uint32_t)nwg), dim3(256), bwd_lds_bytes(2), stream>>>(p);
         else
-            hipLaunchKernelGGL(bwd_tile_kernel<1>, dim3((uint32_t)nwg), dim3(256), bwd_lds_bytes(1), stream, p);
+            bwd_tile_kernel<1, kBI><<<dim3((uint32_t)nwg), dim3(256), bwd_lds_bytes(1), stream>>>(p);
         FL_LAUNCH_CHECK();
     }
     return FL_OK;
@@ -916,6 +931,63 @@ int launch_huber(const float *C, float *grad, int64_t n_slices, int64_t H, int64
     if (n == 0) return FL_OK;
     hipLaunchKernelGGL(huber_kernel, dim3((uint32_t)((n + 255) / 256)), dim3(256), 0, stream, C, grad,
                        n_slices, (int)H, (int)W, delta);
+    FL_LAUNCH_CHECK();
+    return FL_OK;
+}
+
+// Workgroups of `kernel` resident on the current device at once.
+static int resident_grid(const void *kernel, size_t lds, int *grid)
+{
+    int dev = 0, cus = 0, per_cu = 0;
+    FL_HIP_CHECK(hipGetDevice(&dev));
+    FL_HIP_CHECK(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+    FL_HIP_CHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, 256, lds));
+    *grid = std::max(1, cus) * std::max(1, per_cu);
+    return FL_OK;
+}
+
+static const size_t kFwdGroupLds = sizeof(float) * kFLds;
+static const size_t kBwdGroupLds = bwd_lds_bytes(1, 31);  // the larger of the two tile heights
+
+int fwd_group_grid(int *grid)
+{
+    FL_HIP_CHECK(hipFuncSetAttribute((const void *)fwd_group_kernel, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                     (int)kFwdGroupLds));
+    return resident_grid((const void *)fwd_group_kernel, kFwdGroupLds, grid);
+}
+
+int bwd_group_grid(int *grid)
+{
+    FL_HIP_CHECK(hipFuncSetAttribute((const void *)bwd_group_kernel, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                     (int)kBwdGroupLds));
+    return resident_grid((const void *)bwd_group_kernel, kBwdGroupLds, grid);
+}
+
+int copy_group_grid(int *grid) { return resident_grid((const void *)copy_group_kernel, 0, grid); }
+
+int launch_fwd_group(const FwdArgs *jobs, const uint32_t *begin, int n_jobs, uint32_t n_tiles, int grid,
+                     hipStream_t s)
+{
+    if (n_tiles == 0) return FL_OK;
+    fwd_group_kernel<<<dim3((uint32_t)grid), dim3(256), kFwdGroupLds, s>>>(jobs, begin, n_jobs, n_tiles);
+    FL_LAUNCH_CHECK();
+    return FL_OK;
+}
+
+int launch_bwd_group(const BwdArgs *jobs, const uint32_t *begin, int n_jobs, uint32_t n_tiles, int grid,
+                     hipStream_t s)
+{
+    if (n_tiles == 0) return FL_OK;
+    bwd_group_kernel<<<dim3((uint32_t)grid), dim3(256), kBwdGroupLds, s>>>(jobs, begin, n_jobs, n_tiles);
+    FL_LAUNCH_CHECK();
+    return FL_OK;
+}
+
+int launch_copy_group(const CopyJob *jobs, const uint32_t *begin, int n_jobs, uint32_t n_tiles, int grid,
+                      hipStream_t s)
+{
+    if (n_tiles == 0) return FL_OK;
+    copy_group_kernel<<<dim3((uint32_t)grid), dim3(256), 0, s>>>(jobs, begin, n_jobs, n_tiles);
     FL_LAUNCH_CHECK();
     return FL_OK;
 }

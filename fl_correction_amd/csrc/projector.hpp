@@ -1,0 +1,117 @@
+// Tile shapes and launch arguments of the emission projector and its
+// transpose (projector.hip), shared with the batch planner (batch.hip), which
+// fills the same argument records into device job tables.
+#pragma once
+#include "common.hpp"
+
+namespace fl {
+
+// ---------------------------------------------------------------------------
+// Forward projector
+// ---------------------------------------------------------------------------
+constexpr int kFW = 64;                   // detector columns per workgroup (one per lane)
+constexpr int kFH = 32;                   // detector rows per chunk
+constexpr int kFWaves = 4;
+constexpr int kFRows = kFH / kFWaves;     // rows per wave per chunk
+constexpr int kFG = 8;                    // angles per workgroup (one accumulator each)
+constexpr int kFLds = 11264;              // footprint floats staged per workgroup (44 KB)
+// One angle's footprint for a 64x32 tile: the sample points span at most
+// hypot(31, 63) < 70.3 rows (and cols); with the 0.5 margins and the bilinear
+// neighbour that is <= 74 rows and a pitch <= 75.
+constexpr int kFMaxFootprint = 74 * 75;
+constexpr int kFRefChunk = kFLds / kFMaxFootprint;  // references staged together (2)
+
+enum FwdMode { kFwdWrite = 0, kFwdAccumulate = 1, kFwdRatio = 2, kFwdAccumulateRatio = 3 };
+
+struct FwdArgs {
+    const float *atten;
+    const float *C;
+    const float *em;
+    const float *theta;
+    const float *I;  // ratio modes
+    float *out;      // (A, B, W): Pf or ratio
+    int A, B, R, H, W;
+    int r0, nref;
+    int mode;
+    int ntw, n_groups;
+    int group_angles;  // grouped launches: angles per workgroup of this job (4 or 8)
+};
+
+// ---------------------------------------------------------------------------
+// Back-projection (exact transpose) and the fused MLEM update
+// ---------------------------------------------------------------------------
+constexpr int kBI = 31;               // image rows per tile (base cells of its samples: 32 rows)
+constexpr int kBJ = 63;               // image cols per tile (base cells: 64 cols)
+constexpr int kBWaves = 4;            // private accumulators per workgroup
+constexpr int kBPitch = kBJ + 2;      // 65: base-cell cols plus the right neighbour (odd)
+// Accumulator cells of a tile BI rows high: its base cells plus one halo row
+// below and one column to the right.
+constexpr int bwd_acc_cells(int bi) { return (bi + 2) * kBPitch; }
+constexpr int kBRefChunk = 2;
+constexpr int kBSink = 136;           // sink cells: lane + corner offset <= 63 + 66
+constexpr int kBItems = 32;           // items of 128 samples per enumeration window
+constexpr int kBWin = 128 * kBItems;  // samples per window
+constexpr int kBUnroll = 2;           // items splatted together
+constexpr int kBSlots = 64;           // election table for clamped samples (hash of the cell)
+constexpr int kBRows = 48;            // detector rows per batch
+static_assert(kBItems * 2 == 64, "one mask word per lane");
+static_assert(kBItems % kBUnroll == 0, "items come in whole unroll groups");
+
+// Dynamic LDS of the back-projector: four private accumulators (cell-major,
+// nref values per cell), then one sink shared by the waves (it only ever holds
+// discarded values, so their unordered writes there are harmless).  With the
+// static tables this stays under 40 KB for nref = 1: four workgroups per CU.
+constexpr size_t bwd_lds_bytes(int nref, int bi = kBI)
+{
+    return sizeof(float) * (size_t)nref * (kBWaves * bwd_acc_cells(bi) + kBSink);
+}
+
+enum BwdMode { kBwdWrite = 0, kBwdSens = 1, kBwdUpdate = 2, kBwdAccumulate = 3 };
+
+struct BwdArgs {
+    const float *atten;
+    const float *ratio;  // (A,B,W) or null (= 1)
+    const float *em;
+    const float *theta;
+    float *out;  // (R,B,H,W)
+    int A, B, R, H, W;
+    int r0, nref;
+    int nti, ntj;
+    int mode;
+    const float *C;     // update: current estimate (R,B,H,W)
+    const float *sens;  // update: sensitivity (R,B,H,W)
+    float beta, delta;
+    int tile_rows;  // grouped launches: image rows per tile of this job (15 or 31)
+};
+
+// ---------------------------------------------------------------------------
+// Grouped (batched) launches: one persistent grid walks the tiles of many
+// independent jobs.  Job j owns the logical tiles [begin[j], begin[j+1]);
+// begin[0] = 0 and begin[n_jobs] = the total.
+// ---------------------------------------------------------------------------
+constexpr int kCopyTile = 4096;  // floats per tile of the grouped copy
+
+struct CopyJob {
+    const float *src;
+    float *dst;
+    int64_t n;
+};
+
+// Tiles of one job of the grouped launches.
+inline uint32_t fwd_job_tiles(const FwdArgs &p) { return (uint32_t)(p.ntw * p.n_groups * p.B); }
+inline uint32_t bwd_job_tiles(const BwdArgs &p) { return (uint32_t)(p.nti * p.ntj * p.B); }
+inline uint32_t copy_job_tiles(const CopyJob &c) { return (uint32_t)((c.n + kCopyTile - 1) / kCopyTile); }
+
+// Persistent grid sizes (resident workgroups on the current device).
+int fwd_group_grid(int *grid);
+int bwd_group_grid(int *grid);
+int copy_group_grid(int *grid);
+// Enqueue one grouped pass; jobs and begin are device tables.
+int launch_fwd_group(const FwdArgs *jobs, const uint32_t *begin, int n_jobs, uint32_t n_tiles, int grid,
+                     hipStream_t s);
+int launch_bwd_group(const BwdArgs *jobs, const uint32_t *begin, int n_jobs, uint32_t n_tiles, int grid,
+                     hipStream_t s);
+int launch_copy_group(const CopyJob *jobs, const uint32_t *begin, int n_jobs, uint32_t n_tiles, int grid,
+                      hipStream_t s);
+
+}  // namespace fl

@@ -116,6 +116,63 @@ FL_API int fl_mlem_update(const float *atten, const float *ratio, const float *e
                           double delta, void *stream);
 
 /* ---------------------------------------------------------------------------
+ * Batched MLEM: many independent problems of uneven size, each pass of an
+ * iteration one persistent launch over all of them.
+ *
+ * The reference has no batched form: these calls replace a host loop of
+ * mlem_cuda_batch calls (FL/cuda_lib/atten_cuda.cpp:283-345), one per sample
+ * and emission line, each behind torch_mlem_recon_batch (FL/torch_recon.py:50-98).
+ * The iteration is mlem_cuda_batch's, per problem, with R = 1.
+ * ------------------------------------------------------------------------- */
+
+/* One problem: one emission line of one sample.  atten (A,B,H,W), em_cs (A),
+ * theta (A) radians, I (A,B,W), C (B,H,W): the starting estimate, overwritten
+ * with the result by fl_mlem_multi_run.  Tile shapes (0 = default):
+ * fwd_angles, angles per forward workgroup, 4 or 8 (default 8); bwd_rows, image
+ * rows per back-projector tile, 15 or 31 (default 31).  The results do not
+ * depend on fwd_angles; bwd_rows changes only the order of the per-voxel sums. */
+typedef struct fl_mlem_problem {
+    const float *atten;
+    const float *em_cs;
+    const float *theta;
+    const float *I;
+    float *C;
+    int64_t A, B, H, W;
+    int32_t fwd_angles;
+    int32_t bwd_rows;
+} fl_mlem_problem;
+
+/* Host-side plan: job tables (in the workspace), persistent grid sizes and,
+ * once run in graph mode, the instantiated hipGraph of two iterations. */
+typedef struct fl_mlem_multi fl_mlem_multi;
+
+/* Workspace for n problems: job tables, and per problem the sensitivity, a
+ * second estimate (B,H,W) and the ratio (A,B,W).  0 for invalid arguments.
+ * Host-only arithmetic. */
+FL_API size_t fl_mlem_multi_workspace_bytes(const fl_mlem_problem *problems, int64_t n);
+
+/* Validate the problems (shapes, tile shapes, disjoint C), build and upload the
+ * job tables into `workspace` (device memory, 256-byte aligned, of at least
+ * fl_mlem_multi_workspace_bytes bytes) and enqueue the sensitivity pass
+ * sens = max(B^T 1, 1e-6) on `stream`.  The problem arrays (not the struct
+ * array) must stay valid while the plan is used.  Allocates no device memory. */
+FL_API int fl_mlem_multi_create(const fl_mlem_problem *problems, int64_t n, double beta,
+                                double delta, void *workspace, size_t workspace_bytes,
+                                void *stream, fl_mlem_multi **plan);
+
+/* n_iter MLEM (beta <= 0) or OSL-Huber (beta > 0) iterations of every problem,
+ * continuing from its C and leaving the result in C.  use_graph != 0: pairs of
+ * iterations are replays of one hipGraph, captured on `stream` (which must not
+ * be the NULL stream) at the first such call; an odd last iteration runs
+ * eagerly.  Calls on one plan are ordered on their streams; they must not
+ * overlap. */
+FL_API int fl_mlem_multi_run(fl_mlem_multi *plan, int64_t n_iter, int use_graph, void *stream);
+
+/* Release the plan (host memory and its graph) once its work has finished.
+ * The workspace stays the caller's. */
+FL_API int fl_mlem_multi_destroy(fl_mlem_multi *plan);
+
+/* ---------------------------------------------------------------------------
  * Attenuation maps.
  *
  * Frames: the original frame is (B, H, W) = (slice, row, col), incident beam

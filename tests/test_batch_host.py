@@ -1,0 +1,77 @@
+"""Host-side parts of the batched MLEM: workspace layout, argument checks that
+run before any device call, and the autotuner's buckets and cache (no GPU)."""
+import ctypes
+import json
+import os
+
+import pytest
+
+from fl_correction_amd import _lib
+
+
+def _lib_or_skip():
+    if not os.path.exists(_lib.LIB_PATH):
+        pytest.skip("library not built")
+    return _lib.load()
+
+
+def _arr(shapes):
+    arr = (_lib.MLEMProblem * len(shapes))()
+    for i, (A, B, H, W) in enumerate(shapes):
+        arr[i] = _lib.MLEMProblem(1, 1, 1, 1, 1 << 20 + i, A, B, H, W, 0, 0)
+    return arr
+
+
+def test_struct_layout_matches_header():
+    # five pointers, four int64, two int32: 5*8 + 4*8 + 2*4
+    assert ctypes.sizeof(_lib.MLEMProblem) == 80
+    assert _lib.MLEMProblem.A.offset == 40 and _lib.MLEMProblem.fwd_angles.offset == 72
+
+
+def test_workspace_bytes():
+    lib = _lib_or_skip()
+    shapes = [(30, 3, 64, 64), (12, 2, 40, 70), (9, 1, 97, 33)]
+    arr = _arr(shapes)
+    got = lib.fl_mlem_multi_workspace_bytes(ctypes.cast(arr, ctypes.c_void_p), len(shapes))
+    al = lambda b: (b + 255) // 256 * 256
+    per = sum(2 * al(4 * B * H * W) + al(4 * A * B * W) for A, B, H, W in shapes)
+    # tables: 2 forward + 3 back-projector job tables, a copy table, 3 tile prefix arrays
+    assert got > per and got - per < 5 * al(200 * len(shapes)) + al(32 * len(shapes)) + 3 * al(4 * (len(shapes) + 1)) + 1
+    assert got % 256 == 0
+    assert lib.fl_mlem_multi_workspace_bytes(None, 3) == 0
+    assert lib.fl_mlem_multi_workspace_bytes(ctypes.cast(arr, ctypes.c_void_p), 0) == 0
+
+
+def test_create_rejects_before_device_work():
+    lib = _lib_or_skip()
+    plan = ctypes.c_void_p()
+    bad = _arr([(4, 1, 1, 8)])  # H < 2
+    rc = lib.fl_mlem_multi_create(ctypes.cast(bad, ctypes.c_void_p), 1, 0.0, 0.01, None, 0, None, ctypes.byref(plan))
+    assert rc == 1 and b"shape" in lib.fl_last_error()
+    arr = _arr([(4, 1, 8, 8)])
+    arr[0].fwd_angles = 6
+    rc = lib.fl_mlem_multi_create(ctypes.cast(arr, ctypes.c_void_p), 1, 0.0, 0.01, None, 0, None, ctypes.byref(plan))
+    assert rc == 1 and b"tile shape" in lib.fl_last_error()
+    two = _arr([(4, 1, 8, 8), (4, 1, 8, 8)])
+    two[1].C = two[0].C + 4 * 8  # overlaps the first estimate
+    rc = lib.fl_mlem_multi_create(ctypes.cast(two, ctypes.c_void_p), 2, 0.0, 0.01, None, 0, None, ctypes.byref(plan))
+    assert rc == 1 and b"overlap" in lib.fl_last_error()
+    ok = _arr([(4, 1, 8, 8)])
+    rc = lib.fl_mlem_multi_create(ctypes.cast(ok, ctypes.c_void_p), 1, 0.0, 0.01, None, 0, None, ctypes.byref(plan))
+    assert rc == 4 and b"workspace" in lib.fl_last_error()
+    assert not plan.value
+    assert lib.fl_mlem_multi_run(None, 2, 1, None) == 1
+    assert lib.fl_mlem_multi_destroy(None) == 0
+
+
+def test_autotuner_buckets_and_cache(tmp_path):
+    from fl_correction_amd.batch import Autotuner
+    assert Autotuner.bucket(64, 64) == (64, 64)
+    assert Autotuner.bucket(65, 97) == (96, 128)
+    assert Autotuner.key("MI355X", 100, 160) == "MI355X|128x160"
+    path = tmp_path / "t.json"
+    path.write_text(json.dumps({"X|64x64": [4, 15]}))
+    tu = Autotuner(path=str(path))
+    assert tu.table["X|64x64"] == [4, 15]
+    (tmp_path / "bad.json").write_text("{not json")
+    assert Autotuner(path=str(tmp_path / "bad.json")).table == {}

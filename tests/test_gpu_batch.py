@@ -1,0 +1,128 @@
+"""Batched MLEM (fl_mlem_multi_*): many uneven problems in one persistent launch
+per pass, two iterations per replay of a captured hipGraph.
+
+Bars: every problem's result equals the per-problem path (fl_mlem_batch, same
+kernels) to 1e-6 relative L2, and the FP32 oracle to 1e-5; graph replays
+equal eager runs bit for bit (the kernels are deterministic).
+"""
+import numpy as np
+import pytest
+import torch
+
+from helpers import RTOL, oracle, rel_l2
+
+pytestmark = pytest.mark.gpu
+
+# (A, B, H, W): uneven, ragged against the tile sizes (64-column strips,
+# 31 x 63 and 15 x 63 tiles), one single-slice problem, one tiny one
+SHAPES = [(30, 3, 64, 64), (12, 2, 40, 70), (9, 1, 97, 33), (24, 4, 32, 32), (5, 2, 130, 66), (7, 1, 8, 8)]
+
+
+def _problem(A, B, H, W, seed):
+    rng = np.random.default_rng(seed)
+    atten = rng.uniform(0.3, 1.0, (A, B, H, W)).astype(np.float32)
+    em = rng.uniform(0.5, 2.0, A).astype(np.float32)
+    theta = rng.uniform(0, np.pi, A).astype(np.float32)
+    C_true = rng.uniform(0.0, 1.0, (1, B, H, W)).astype(np.float32)
+    I = oracle.forward_batch_f32(atten, C_true, em[:, None], theta)
+    C0 = np.ones((B, H, W), np.float32)
+    return atten, em, theta, I, C0
+
+
+def _dev(x):
+    return torch.from_numpy(np.ascontiguousarray(x)).cuda()
+
+
+@pytest.fixture(scope="module")
+def data():
+    return [_problem(*s, seed=100 + i) for i, s in enumerate(SHAPES)]
+
+
+def _batch(data, shapes=None, beta=0.0):
+    from fl_correction_amd.batch import BatchMLEM, Problem
+    probs = [Problem(_dev(a), _dev(e), _dev(t), _dev(I), _dev(C0)) for a, e, t, I, C0 in data]
+    return BatchMLEM(probs, beta=beta, delta=0.01, shapes=shapes), probs
+
+
+def _single(d, n_iter, beta):
+    from fl_correction_amd.cuda_lib import atten_cuda
+    a, e, t, I, C0 = d
+    out = atten_cuda.mlem_cuda_batch(_dev(C0[None]), _dev(a), _dev(e[:, None]), _dev(t), _dev(I), n_iter, beta, 0.01)
+    return out[0].cpu().numpy()
+
+
+@pytest.mark.parametrize("beta", [0.0, 1e-3])
+@pytest.mark.parametrize("n_iter", [3, 4])
+def test_batch_matches_single_and_oracle(data, beta, n_iter):
+    run, probs = _batch(data, beta=beta)
+    print(f"batch beta={beta} n_iter={n_iter}: {len(probs)} problems", flush=True)
+    run.run(n_iter, graph=True)
+    torch.cuda.synchronize()
+    for i, (d, p) in enumerate(zip(data, probs)):
+        got = p.C.cpu().numpy()
+        ref1 = _single(d, n_iter, beta)
+        a, e, t, I, C0 = d
+        ref = oracle.mlem_batch_f32(C0[None], a, e[:, None], t, I, n_iter, beta, 0.01)[0]
+        e1, e32 = rel_l2(got, ref1), rel_l2(got, ref)
+        print(f"  problem {i} {SHAPES[i]}: vs single {e1:.1e}, vs f32 oracle {e32:.1e}")
+        assert e1 <= 1e-6
+        assert e32 <= RTOL
+    run.close()
+
+
+def test_tile_shapes_agree(data):
+    runs = []
+    for shp in [(8, 31), (4, 31), (8, 15), (4, 15)]:
+        run, probs = _batch(data, shapes=[shp] * len(data))
+        run.run(4, graph=True)
+        torch.cuda.synchronize()
+        runs.append([p.C.cpu().numpy() for p in probs])
+        run.close()
+    for k in range(1, 4):
+        for i in range(len(data)):
+            assert rel_l2(runs[k][i], runs[0][i]) <= 1e-6, (k, i)
+    # forward angles per workgroup change no arithmetic
+    for i in range(len(data)):
+        assert np.array_equal(runs[1][i], runs[0][i])
+
+
+def test_graph_replays_equal_eager(data):
+    g_run, g_probs = _batch(data, beta=1e-3)
+    e_run, e_probs = _batch(data, beta=1e-3)
+    side = torch.ones(1 << 20, device="cuda")
+    for rep in range(3):
+        g_run.run(2, graph=True)  # one replay each time
+        side.mul_(1.0001)          # eager work of the program between replays
+        torch.cuda.synchronize()
+        e_run.run(2, graph=False)
+        torch.cuda.synchronize()
+        for gp, ep in zip(g_probs, e_probs):
+            assert torch.equal(gp.C, ep.C), f"replay {rep}"
+    g_run.close()
+    e_run.close()
+
+
+def test_batch_errors(data):
+    from fl_correction_amd import _lib
+    from fl_correction_amd.batch import BatchMLEM, Problem
+    a, e, t, I, C0 = data[0]
+    p = Problem(_dev(a), _dev(e), _dev(t), _dev(I), _dev(C0))
+    with pytest.raises(_lib.FLError, match="tile shape"):
+        BatchMLEM([p], shapes=[(5, 31)])
+    q = Problem(p.atten, p.em_cs, p.theta, p.I, p.C)  # same estimate twice
+    with pytest.raises(_lib.FLError, match="overlap"):
+        BatchMLEM([p, q])
+    with pytest.raises(ValueError):
+        BatchMLEM([Problem(p.atten, p.em_cs, p.theta, p.I, p.C[:1])])
+
+
+def test_autotuner_picks_and_caches(tmp_path):
+    from fl_correction_amd.batch import Autotuner, BWD_ROWS, FWD_ANGLES
+    path = tmp_path / "tune.json"
+    tu = Autotuner(path=str(path), reps=1, tune_angles=24)
+    fa, br = tu.shapes(24, 2, 40, 40)
+    assert fa in FWD_ANGLES and br in BWD_ROWS
+    print("autotune 64x64 bucket:", tu.timings)
+    assert len(tu.timings) == len(FWD_ANGLES) * len(BWD_ROWS)
+    again = Autotuner(path=str(path))
+    assert again.shapes(30, 1, 33, 64) == (fa, br)  # same bucket, from the cache
