@@ -50,7 +50,8 @@ def parse():
     ap.add_argument("--batch-iters", type=int, default=20)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--calibrate-copy", action="store_true",
-                    help="first copy 4 GiB device to device (a kernel of known traffic, to scale HBM counters)")
+                    help="first run two elementwise kernels of known traffic (4 GiB read + 4 GiB written, with "
+                         "16-byte and with 4-byte accesses), to scale HBM counters")
     return ap.parse_args()
 
 
@@ -80,6 +81,9 @@ def main():
         # an elementwise kernel that reads and writes exactly 4 GiB (a plain
         # copy_ of contiguous tensors is a DMA transfer, not a kernel)
         torch.mul(src, 2.0, out=dst)
+        # the same with 4-byte accesses: a base one float off 16-byte alignment
+        # makes torch fall back to its scalar elementwise kernel
+        torch.mul(src[1:], 2.0, out=dst[1:])
         torch.cuda.synchronize()
         del src, dst
     N, A, K = args.size, args.angles, args.elements

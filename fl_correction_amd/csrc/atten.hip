@@ -205,59 +205,89 @@ __global__ __launch_bounds__(256) void mu_rot_kernel(const float *__restrict__ f
 // reverse = 1 runs the transpose (the adjoint used by the gradient): taps with
 // negated offsets, chains walked from the near end, J(v) = S(v) + J(v - D).
 // ---------------------------------------------------------------------------
-constexpr int kMaxTaps = 128;
+//
+// One thread per chain: lanes take 64 neighbouring chains along y, each wave
+// one z.  Every tap of the q-step stencil sits at a constant offset from the
+// chain point (dz*X*Y + t*Y + dy floats), so a step issues all taps as
+// independent bounds-checked buffer loads from one per-lane base (no per-tap
+// address arithmetic, no branches), summed in four partial sums.  Lanes whose
+// stencil lies wholly outside the volume in y get a base past the buffer (their
+// loads read 0).  A wave with a lane whose stencil straddles a y edge, or whose
+// stencil leaves the volume in z or x, checks every tap instead; such steps
+// occur only where chains cross the volume's faces.  Steps where the wave's
+// stencils and chain points all lie outside are skipped (nothing to add or
+// write).  The stencil table is padded to a power of two with zero-weight taps
+// at offset 0.
+// ---------------------------------------------------------------------------
+constexpr int kConeMaxTaps = 128;
+// Base byte offset that stays at or past every buffer end (buffers hold < 2^31
+// bytes) for every tap offset (|offset| < 2^29 bytes), without wrapping.
+constexpr uint32_t kConeOOB = 0xC0000000u;
 
-struct ConeTaps {
-    int n;
+struct ConeArgs {
     int pz, py, q;
     float weight;
     int accumulate;
     int reverse;
-    int zlo, nz, ylo, ny;  // chain start ranges (mask frame)
-    int8_t t[kMaxTaps];
-    int8_t dz[kMaxTaps];
-    int8_t dy[kMaxTaps];
-    float w[kMaxTaps];
+    int zlo, nz, ylo, ny;           // chain start ranges (mask frame)
+    int dzmin, dzmax, dymin, dymax;  // stencil extent
+    int tmin, tmax;
+    int ezlo, ezhi, eylo, eyhi;      // stencil extent including the chain point itself
+    int off[kConeMaxTaps];           // tap byte offsets
+    int8_t t[kConeMaxTaps];
+    int8_t dz[kConeMaxTaps];
+    int8_t dy[kConeMaxTaps];
+    float w[kConeMaxTaps];
 };
 
-__global__ __launch_bounds__(256) void cone_kernel(const float *__restrict__ mu, float *__restrict__ T, int K, int Z,
-                                                   int X, int Y, ConeTaps p)
+template <int NT>
+__global__ __launch_bounds__(256) void cone_kernel(const float *__restrict__ mu, float *__restrict__ T, int Z, int X,
+                                                   int Y, ConeArgs p)
 {
-    const int yi = blockIdx.x * 64 + (threadIdx.x & 63);
-    const int zi = blockIdx.y * 4 + (threadIdx.x >> 6);
+    const int lane = threadIdx.x & 63;
+    const int zi = blockIdx.y * 4 + uniform_int(threadIdx.x >> 6);
+    if (zi >= p.nz) return;  // the whole wave
+    const int yi = blockIdx.x * 64 + lane;
+    const bool act = yi < p.ny;
     const int x0 = blockIdx.z % p.q;
     const int k = blockIdx.z / p.q;
-    if (yi >= p.ny || zi >= p.nz) return;
-    const int z0 = p.zlo + zi, y0 = p.ylo + yi;
-    const int64_t XY = (int64_t)X * Y;
-    const float *muk = mu + (int64_t)k * Z * XY;
-    float *Tk = T + (int64_t)k * Z * XY;
-    const int M = (X - x0 + p.q - 1) / p.q;  // chain points with x < X
+    const int64_t ZXY = (int64_t)Z * X * Y;
+    const __amdgpu_buffer_rsrc_t rs = make_rsrc(mu + k * ZXY, (int)ZXY);
+    float *Tk = T + k * ZXY;
+    const int z0 = p.zlo + zi;
+    const int yw0 = p.ylo + (int)blockIdx.x * 64;  // lane 0's chain start
+    const int M = (X - x0 + p.q - 1) / p.q;        // chain points with x < X
     float J = 0.0f;
     for (int it = 0; it < M; ++it) {
         const int m = p.reverse ? it : M - 1 - it;
         const int x = x0 + m * p.q;
         const int z = z0 + m * p.pz;
-        const int y = y0 + m * p.py;
-        // whole stencil outside the transverse range (plus the bilinear reach)?
-        const int zmin = z - 2 * (abs(p.pz) + 1), zmax = z + 2 * (abs(p.pz) + 1);
-        const int ymin = y - 2 * (abs(p.py) + 1), ymax = y + 2 * (abs(p.py) + 1);
-        if (zmax < 0 || zmin >= Z || ymax < 0 || ymin >= Y) {
-            // the ray from here stays outside: only J beyond matters, which is
-            // already in J (zero if the chain has not entered yet)
-            continue;
+        const int yw = yw0 + m * p.py;
+        const int y = yw + lane;
+        if (z + p.ezhi < 0 || z + p.ezlo >= Z || yw + 63 + p.eyhi < 0 || yw + p.eylo >= Y) continue;
+        const bool yin = act && y + p.dymin >= 0 && y + p.dymax < Y;
+        const bool yout = !act || y + p.dymax < 0 || y + p.dymin >= Y;
+        const bool zx_in = z + p.dzmin >= 0 && z + p.dzmax < Z && x + p.tmin >= 0 && x + p.tmax < X;
+        // chain point as a float index; wraps harmlessly where it is not used
+        const uint32_t base = ((uint32_t)z * (uint32_t)X + (uint32_t)x) * (uint32_t)Y + (uint32_t)y;
+        float s[4] = {0.0f, 0.0f, 0.0f, 0.0f};
+        if (zx_in && __all(yin || yout)) {
+            const uint32_t vb = yin ? base * 4u : kConeOOB;
+#pragma unroll
+            for (int i = 0; i < NT; ++i) s[i & 3] = fmaf(p.w[i], load_f32(rs, vb + (uint32_t)p.off[i]), s[i & 3]);
+        } else {
+#pragma unroll
+            for (int i = 0; i < NT; ++i) {
+                const int zz = z + p.dz[i], xx = x + p.t[i], yy = y + p.dy[i];
+                const bool ok = act & ((uint32_t)zz < (uint32_t)Z) & ((uint32_t)xx < (uint32_t)X) &
+                                ((uint32_t)yy < (uint32_t)Y);
+                const uint32_t v = ok ? base * 4u + (uint32_t)p.off[i] : kConeOOB;
+                s[i & 3] = fmaf(p.w[i], load_f32(rs, v), s[i & 3]);
+            }
         }
-        float S = 0.0f;
-        for (int i = 0; i < p.n; ++i) {
-            const int xx = x + p.t[i];
-            const int zz = z + p.dz[i];
-            const int yy = y + p.dy[i];
-            if (xx >= 0 && xx < X && zz >= 0 && zz < Z && yy >= 0 && yy < Y)
-                S = fmaf(p.w[i], muk[(int64_t)zz * XY + (int64_t)xx * Y + yy], S);
-        }
-        J += S;
-        if (z >= 0 && z < Z && y >= 0 && y < Y) {
-            float *o = Tk + (int64_t)z * XY + (int64_t)x * Y + y;
+        J += (s[0] + s[1]) + (s[2] + s[3]);
+        if (act && z >= 0 && z < Z && y >= 0 && y < Y) {
+            float *o = Tk + base;
             *o = p.accumulate ? fmaf(p.weight, J, *o) : p.weight * J;
         }
     }
@@ -365,9 +395,10 @@ int launch_cone_direction(const float *mu, float *T, int64_t K, int64_t Z, int64
         py /= g;
         q /= g;
     }
-    FL_REQUIRE(4 * q <= kMaxTaps, "cone direction: reduced denominator too large (max 32)");
+    FL_REQUIRE(4 * q <= kConeMaxTaps, "cone direction: reduced denominator too large (max 32)");
     FL_REQUIRE(std::abs(pz) <= 64 * q && std::abs(py) <= 64 * q, "cone direction: slope too steep");
-    ConeTaps p{};
+    FL_REQUIRE(Z * X * Y < ((int64_t)1 << 29), "cone: one line's volume must hold fewer than 2^29 voxels");
+    ConeArgs p{};
     p.pz = pz;
     p.py = py;
     p.q = q;
@@ -376,6 +407,8 @@ int launch_cone_direction(const float *mu, float *T, int64_t K, int64_t Z, int64
     p.reverse = reverse;
     const int sg = reverse ? -1 : 1;
     int n = 0;
+    p.dzmin = p.dymin = p.tmin = INT32_MAX;
+    p.dzmax = p.dymax = p.tmax = INT32_MIN;
     for (int t = 1; t <= q; ++t) {
         const int64_t nz = (int64_t)t * pz, ny = (int64_t)t * py;
         const int64_t z0 = floordiv(nz, q), y0 = floordiv(ny, q);
@@ -384,16 +417,32 @@ int launch_cone_direction(const float *mu, float *T, int64_t K, int64_t Z, int64
             for (int b = 0; b < 2; ++b) {
                 const double wz = a ? fz : 1.0 - fz, wy = b ? fy : 1.0 - fy;
                 if (wz == 0.0 || wy == 0.0) continue;
-                FL_REQUIRE(n < kMaxTaps && std::abs(z0 + a) < 127 && std::abs(y0 + b) < 127,
+                FL_REQUIRE(n < kConeMaxTaps && std::abs(z0 + a) < 127 && std::abs(y0 + b) < 127,
                            "cone direction: stencil too large");
-                p.t[n] = (int8_t)(sg * t);
-                p.dz[n] = (int8_t)(sg * (z0 + a));
-                p.dy[n] = (int8_t)(sg * (y0 + b));
+                const int tt = sg * t, dz = (int)(sg * (z0 + a)), dy = (int)(sg * (y0 + b));
+                const int64_t off = ((int64_t)dz * X * Y + (int64_t)tt * Y + dy) * 4;
+                FL_REQUIRE(std::llabs(off) < ((int64_t)1 << 29), "cone direction: stencil reach too large");
+                p.t[n] = (int8_t)tt;
+                p.dz[n] = (int8_t)dz;
+                p.dy[n] = (int8_t)dy;
+                p.off[n] = (int)off;
                 p.w[n] = (float)(wz * wy);
+                p.dzmin = std::min(p.dzmin, dz);
+                p.dzmax = std::max(p.dzmax, dz);
+                p.dymin = std::min(p.dymin, dy);
+                p.dymax = std::max(p.dymax, dy);
+                p.tmin = std::min(p.tmin, tt);
+                p.tmax = std::max(p.tmax, tt);
                 ++n;
             }
     }
-    p.n = n;
+    p.ezlo = std::min(p.dzmin, 0);
+    p.ezhi = std::max(p.dzmax, 0);
+    p.eylo = std::min(p.dymin, 0);
+    p.eyhi = std::max(p.dymax, 0);
+    int NT = 4;
+    while (NT < n) NT *= 2;
+    // the padding taps (weight 0, offset 0) read the chain point itself
     const int64_t Mmax = (X + q - 1) / q;
     // chain starts in plane x0 in [0,q): z0 = z - m*pz for z in [0,Z), m in [0,Mmax);
     // one extra voxel each side for the bilinear reach
@@ -404,7 +453,15 @@ int launch_cone_direction(const float *mu, float *T, int64_t K, int64_t Z, int64
     p.ylo = (int)ylo;
     p.ny = (int)(yhi - ylo + 1);
     dim3 grid((unsigned)((p.ny + 63) / 64), (unsigned)((p.nz + 3) / 4), (unsigned)(K * q));
-    hipLaunchKernelGGL(cone_kernel, grid, dim3(256), 0, s, mu, T, (int)K, (int)Z, (int)X, (int)Y, p);
+    FL_REQUIRE(K * q <= 65535 && (p.nz + 3) / 4 <= 65535, "cone direction: grid too large");
+    switch (NT) {
+    case 4: cone_kernel<4><<<grid, dim3(256), 0, s>>>(mu, T, (int)Z, (int)X, (int)Y, p); break;
+    case 8: cone_kernel<8><<<grid, dim3(256), 0, s>>>(mu, T, (int)Z, (int)X, (int)Y, p); break;
+    case 16: cone_kernel<16><<<grid, dim3(256), 0, s>>>(mu, T, (int)Z, (int)X, (int)Y, p); break;
+    case 32: cone_kernel<32><<<grid, dim3(256), 0, s>>>(mu, T, (int)Z, (int)X, (int)Y, p); break;
+    case 64: cone_kernel<64><<<grid, dim3(256), 0, s>>>(mu, T, (int)Z, (int)X, (int)Y, p); break;
+    default: cone_kernel<128><<<grid, dim3(256), 0, s>>>(mu, T, (int)Z, (int)X, (int)Y, p); break;
+    }
     FL_LAUNCH_CHECK();
     return FL_OK;
 }

@@ -115,6 +115,38 @@ def test_cone_equals_splat_mask_and_march():
     assert rel_l2(via_mask, oracle.atten_f32(mu, M)) <= RTOL
 
 
+@pytest.mark.parametrize("n,half,q,shape", [
+    (5, 15.0, 8, (2, 20, 40, 150)),    # q = 8, 4, 1 stencils; y wider than one wave (unchecked path)
+    (3, 40.0, 4, (1, 13, 33, 200)),    # slopes near 1: chains leave through the z and y faces
+    (3, 20.0, 16, (1, 9, 50, 130)),    # 64-tap stencils
+])
+def test_cone_wide_volumes(n, half, q, shape):
+    from fl_correction_amd import _lib
+    from fl_correction_amd.detector_mask import Cone
+    rng = np.random.default_rng(16)
+    mu = rng.uniform(0, 0.05, shape).astype(np.float32)
+    K, Z, X, Y = shape
+    cone = Cone(n=n, half_angle_deg=half, q=q)
+    args = [np.ascontiguousarray(a) for a in (cone.pz, cone.py, cone.qd, cone.weight)]
+    ptrs = [a.ctypes.data_as(_lib.ctypes.c_void_p) for a in args]
+    mu_d = _dev(mu)
+    T = torch.empty_like(mu_d)
+    print(f"cone n={n} half={half} q={q} shape={shape}", flush=True)
+    _lib.call("fl_cone_atten", _lib.ptr(mu_d), _lib.ptr(T), K, Z, X, Y, *ptrs, cone.ndir, _lib.stream_of(mu_d.device))
+    got = T.cpu().numpy()
+    ref64 = oracle.cone_march_f64(mu, cone.pz, cone.py, cone.qd, cone.weight.astype(np.float64))
+    print(f"  vs f64 march {rel_l2(got, ref64):.2e}")
+    assert rel_l2(got, ref64) <= RTOL
+    # the transpose: <K x, y> = <x, K^T y>
+    y = _dev(rng.uniform(-1, 1, shape).astype(np.float32))
+    Kty = torch.empty_like(y)
+    _lib.call("fl_cone_atten_adjoint", _lib.ptr(y), _lib.ptr(Kty), K, Z, X, Y, *ptrs, cone.ndir,
+              _lib.stream_of(mu_d.device))
+    lhs = float((T.double() * y.double()).sum())
+    rhs = float((mu_d.double() * Kty.double()).sum())
+    assert abs(lhs - rhs) <= 1e-5 * abs(lhs)
+
+
 def test_total_atten_pipeline_against_reference_composition():
     from fl_correction_amd import core
     from fl_correction_amd.atten_util import make_param
