@@ -164,9 +164,11 @@ def main():
     upd_bytes = 4.0 * (A * n3 + A * B * W + 3 * R * n3)      # atten, ratio, C, sens in, C_next out
     iter_bytes = K * (fwd_bytes + upd_bytes)
     if upd_avg >= fwd_avg:
-        dom, dom_ms, dom_bytes = "bwd_tile_kernel<1> (back-projection + MLEM update)", upd_avg, upd_bytes
+        dom, dom_ms, dom_bytes = "bwd_tile_kernel<1,31> (back-projection + MLEM update)", upd_avg, upd_bytes
+        dom_key = "bwd_tile_kernel<1,31>"
     else:
-        dom, dom_ms, dom_bytes = "fwd_strip_kernel<1> (forward projection + ratio)", fwd_avg, fwd_bytes
+        dom, dom_ms, dom_bytes = "fwd_strip_kernel<1,8> (forward projection + ratio)", fwd_avg, fwd_bytes
+        dom_key = "fwd_strip_kernel<1,8>"
     achieved = dom_bytes / (dom_ms * 1e-3) / 1e9
 
     result = {
@@ -188,7 +190,7 @@ def main():
         "stage_ms": {"forward_ratio_per_line": fwd_avg, "backproject_update_per_line": upd_avg},
         "hbm_algorithmic_gbs_per_iteration": iter_bytes / (ms_per_step * 1e-3) / 1e9,
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": measured_traffic(dom_key, N, A),
                      "bytes_per_launch": dom_bytes, "avg_launch_ms": dom_ms},
         "wall_s_timed": wall,
     }
@@ -207,6 +209,22 @@ def main():
         result["cpu_baseline"] = cpu_baseline(args)
 
     print(json.dumps(result), flush=True)
+
+
+def measured_traffic(kernel, N, A):
+    """HBM bytes per launch of `kernel` from the committed rocprofv3 counter
+    summary (read and write requests by size class, see the file), for the
+    256^3 x 360-angle workload it was measured on; None for other sizes."""
+    import re
+    if (N, A) != (256, 360):
+        return None
+    path = os.path.join(ROOT, "profiles", "mlem_iteration_256cubed_360x4_hbm_counters.md")
+    try:
+        text = open(path).read()
+    except OSError:
+        return None
+    m = re.search(r"^traffic " + re.escape(kernel) + r": ([0-9.e+]+) bytes per launch", text, re.M)
+    return float(m.group(1)) if m else None
 
 
 def model_timings(args, C_true, pix):

@@ -254,6 +254,7 @@ __global__ __launch_bounds__(256) void cone_kernel(const float *__restrict__ mu,
     const int64_t ZXY = (int64_t)Z * X * Y;
     const __amdgpu_buffer_rsrc_t rs = make_rsrc(mu + k * ZXY, (int)ZXY);
     float *Tk = T + k * ZXY;
+    const __amdgpu_buffer_rsrc_t rt = make_rsrc(Tk, (int)ZXY);
     const int z0 = p.zlo + zi;
     const int yw0 = p.ylo + (int)blockIdx.x * 64;  // lane 0's chain start
     const int M = (X - x0 + p.q - 1) / p.q;        // chain points with x < X
@@ -270,6 +271,9 @@ __global__ __launch_bounds__(256) void cone_kernel(const float *__restrict__ mu,
         const bool zx_in = z + p.dzmin >= 0 && z + p.dzmax < Z && x + p.tmin >= 0 && x + p.tmax < X;
         // chain point as a float index; wraps harmlessly where it is not used
         const uint32_t base = ((uint32_t)z * (uint32_t)X + (uint32_t)x) * (uint32_t)Y + (uint32_t)y;
+        // the running sum of earlier directions, loaded with the taps
+        const bool wr = act & ((uint32_t)z < (uint32_t)Z) & ((uint32_t)y < (uint32_t)Y);
+        const float tprev = p.accumulate ? load_f32(rt, wr ? base * 4u : kConeOOB) : 0.0f;
         float s[4] = {0.0f, 0.0f, 0.0f, 0.0f};
         if (zx_in && __all(yin || yout)) {
             const uint32_t vb = yin ? base * 4u : kConeOOB;
@@ -286,10 +290,7 @@ __global__ __launch_bounds__(256) void cone_kernel(const float *__restrict__ mu,
             }
         }
         J += (s[0] + s[1]) + (s[2] + s[3]);
-        if (act && z >= 0 && z < Z && y >= 0 && y < Y) {
-            float *o = Tk + base;
-            *o = p.accumulate ? fmaf(p.weight, J, *o) : p.weight * J;
-        }
+        if (wr) Tk[base] = p.accumulate ? fmaf(p.weight, J, tprev) : p.weight * J;
     }
 }
 
