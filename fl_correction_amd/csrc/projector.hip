@@ -342,13 +342,12 @@ struct BwdTile {
 // cell; they are added afterwards, one lane per cell per round, that lane
 // elected by an integer minimum.
 struct BwdCursor {
-    int a, stage;  // stage 0: set up angle a; 1: set up the batch at rb; 2: inside a window
+    int a;
     int cls, rb, hlo, hhi;
     int T, nne, win0, nit, n, kb;
     int P;    // this lane's row: first sample number (batch setup)
     bool ne;  // this lane's row is not empty
     float c, s;
-    bool has_ratio;
     __amdgpu_buffer_rsrc_t ra, rr;
     float ew[kBRefChunk];
 };
@@ -386,7 +385,6 @@ __device__ __forceinline__ void bwd_begin_angle(const BwdArgs &p, const BwdTile 
     cur.s = s;
     const int64_t sl = (int64_t)cur.a * p.B + t.b;
     cur.ra = make_rsrc(p.atten + sl * p.H * p.W, p.H * p.W);
-    cur.has_ratio = p.ratio != nullptr;
     cur.rr = make_rsrc(p.ratio ? p.ratio + sl * p.W : p.atten, p.ratio ? p.W : 0);
 #pragma unroll
     for (int rr = 0; rr < NR; ++rr) cur.ew[rr] = scalar_read(p.em, (int64_t)cur.a * p.R + p.r0 + rr);
@@ -449,61 +447,10 @@ __device__ __forceinline__ void bwd_begin_window(const BwdTile &t, BwdCursor &cu
     cur.n = 0;
 }
 
-// Move to the next item pair of the stream, setting up angles, batches and
-// windows on the way.  False at the end of the stream.
-template <int NR>
-__device__ __forceinline__ bool bwd_next(const BwdArgs &p, const BwdTile &t, BwdCursor &cur)
-{
-    while (true) {
-        if (cur.stage == 2) {
-            cur.n += kBUnroll;
-            if (cur.n < cur.nit) return true;
-            cur.win0 += kBWin;
-            if (cur.win0 < cur.T) {
-                bwd_begin_window(t, cur);
-                return true;
-            }
-            cur.rb += 2 * kBRows;
-            cur.stage = 1;
-        } else if (cur.stage == 1) {
-            if (cur.rb > cur.hhi) {
-                if (cur.cls == 0) {
-                    cur.cls = 1;
-                    cur.rb = cur.hlo + 1;
-                } else {
-                    cur.a += kBWaves;
-                    cur.stage = 0;
-                }
-                continue;
-            }
-            bwd_begin_batch(p, t, cur);
-            if (cur.T == 0) {
-                cur.rb += 2 * kBRows;
-                continue;
-            }
-            cur.win0 = 0;
-            bwd_begin_window(t, cur);
-            cur.stage = 2;
-            return true;
-        } else {
-            if (cur.a >= p.A) {
-                // past the end: leave indices that a dropped gather may still use in range
-                cur.n = 0;
-                cur.nne = max(cur.nne, 1);
-                return false;
-            }
-            bwd_begin_angle<NR>(p, t, cur);
-            cur.cls = 0;
-            cur.rb = cur.hlo;
-            cur.stage = 1;
-        }
-    }
-}
-
 // Positions of the current item pair's samples, and their atten and ratio
 // loads (bounds-checked: an offset past the slice reads 0, so lanes without a
 // sample need no branch; their values go to the sink).
-template <int NR>
+template <int NR, bool RATIO>
 __device__ __forceinline__ void bwd_gather(const BwdArgs &p, const BwdTile &t, BwdCursor &cur, BwdPair &g)
 {
     const int lane = t.lane;
@@ -527,7 +474,7 @@ __device__ __forceinline__ void bwd_gather(const BwdArgs &p, const BwdTile &t, B
         const uint32_t oa = (uint32_t)(rw.x + q) * 4u;
         g.av[u][0] = load_f32(cur.ra, oa);
         g.av[u][1] = load_f32(cur.ra, oa + 4u);
-        if (cur.has_ratio) {
+        if (RATIO) {
             const uint32_t orat = (uint32_t)q * 4u;
             g.rv[u][0] = load_f32(cur.rr, orat);
             g.rv[u][1] = load_f32(cur.rr, orat + 4u);
@@ -637,30 +584,39 @@ __device__ __forceinline__ void bwd_splat(const BwdArgs &p, const BwdTile &t, co
     }
 }
 
-// The wave's whole stream, software-pipelined: the loads of the next item pair
-// are issued before the current pair is splatted (two pair buffers, used in
-// turn, so no register copy waits on a load in flight).  Past the end of the
-// stream the cursor stays on its last window and one more pair is gathered and
-// dropped: the tables it reads are still valid and its loads are
-// bounds-checked.
-template <int NR, bool BORDER>
+// The wave's whole stream: angles a = wave, wave+4, ...; per angle the two row
+// classes, their batches and the batches' windows.  Inside a window the loads
+// of the next item pair are issued before the current pair is splatted (two
+// pair buffers, used in turn, so no register copy waits on a load in flight).
+// RATIO: ratio is given (false: ratio = 1, the sensitivity).
+template <int NR, bool BORDER, bool RATIO>
 __device__ __forceinline__ void bwd_stream(const BwdArgs &p, const BwdTile &t, int wave)
 {
     BwdCursor cur;
-    cur.a = wave;
-    cur.stage = 0;
-    if (!bwd_next<NR>(p, t, cur)) return;
-    BwdPair g0, g1;
-    bwd_gather<NR>(p, t, cur, g0);
-    while (true) {
-        bool more = bwd_next<NR>(p, t, cur);
-        bwd_gather<NR>(p, t, cur, g1);
-        bwd_splat<NR, BORDER>(p, t, g0);
-        if (!more) break;
-        more = bwd_next<NR>(p, t, cur);
-        bwd_gather<NR>(p, t, cur, g0);
-        bwd_splat<NR, BORDER>(p, t, g1);
-        if (!more) break;
+    for (cur.a = wave; cur.a < p.A; cur.a += kBWaves) {
+        bwd_begin_angle<NR>(p, t, cur);
+        for (cur.cls = 0; cur.cls < 2; ++cur.cls) {
+            for (cur.rb = cur.hlo + cur.cls; cur.rb <= cur.hhi; cur.rb += 2 * kBRows) {
+                bwd_begin_batch(p, t, cur);
+                for (cur.win0 = 0; cur.win0 < cur.T; cur.win0 += kBWin) {
+                    bwd_begin_window(t, cur);
+                    BwdPair g0, g1;
+                    bwd_gather<NR, RATIO>(p, t, cur, g0);
+                    while (true) {
+                        cur.n += kBUnroll;
+                        const bool more = cur.n < cur.nit;
+                        if (more) bwd_gather<NR, RATIO>(p, t, cur, g1);
+                        bwd_splat<NR, BORDER>(p, t, g0);
+                        if (!more) break;
+                        cur.n += kBUnroll;
+                        const bool more2 = cur.n < cur.nit;
+                        if (more2) bwd_gather<NR, RATIO>(p, t, cur, g0);
+                        bwd_splat<NR, BORDER>(p, t, g1);
+                        if (!more2) break;
+                    }
+                }
+            }
+        }
     }
 }
 
@@ -723,10 +679,17 @@ __device__ __forceinline__ void bwd_tile(const BwdArgs &p, uint32_t id)
     const bool border = t.lox < 0.0f || t.hix > (float)p.H || t.loy < 0.0f || t.hiy > (float)p.W;
 
     // Wave w takes angles w, w+4, ... into its own accumulator.
-    if (border)
-        bwd_stream<NR, true>(p, t, wave);
-    else
-        bwd_stream<NR, false>(p, t, wave);
+    if (p.ratio) {
+        if (border)
+            bwd_stream<NR, true, true>(p, t, wave);
+        else
+            bwd_stream<NR, false, true>(p, t, wave);
+    } else {
+        if (border)
+            bwd_stream<NR, true, false>(p, t, wave);
+        else
+            bwd_stream<NR, false, false>(p, t, wave);
+    }
     __syncthreads();
 
     // Sum the four private accumulators in a fixed order and write the tile (lane = column).
