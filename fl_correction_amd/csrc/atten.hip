@@ -244,6 +244,14 @@ template <int NT>
 __global__ __launch_bounds__(256) void cone_kernel(const float *__restrict__ mu, float *__restrict__ T, int Z, int X,
                                                    int Y, ConeArgs p)
 {
+    // The tap table, staged once per workgroup and read back every step (one
+    // broadcast LDS read per tap): kept in registers across the walk it would
+    // not fit the scalar registers and be spilled.
+    __shared__ int4 tab[NT];  // (byte offset, weight bits, dy, dz | t << 16)
+    if ((int)threadIdx.x < NT)
+        tab[threadIdx.x] = make_int4(p.off[threadIdx.x], __float_as_int(p.w[threadIdx.x]), p.dy[threadIdx.x],
+                                     (p.dz[threadIdx.x] & 0xffff) | (p.t[threadIdx.x] << 16));
+    __syncthreads();
     const int lane = threadIdx.x & 63;
     const int zi = blockIdx.y * 4 + uniform_int(threadIdx.x >> 6);
     if (zi >= p.nz) return;  // the whole wave
@@ -252,7 +260,8 @@ __global__ __launch_bounds__(256) void cone_kernel(const float *__restrict__ mu,
     const int x0 = blockIdx.z % p.q;
     const int k = blockIdx.z / p.q;
     const int64_t ZXY = (int64_t)Z * X * Y;
-    const __amdgpu_buffer_rsrc_t rs = make_rsrc(mu + k * ZXY, (int)ZXY);
+    const float *muk = mu + k * ZXY;
+    const __amdgpu_buffer_rsrc_t rs = make_rsrc(muk, (int)ZXY);
     float *Tk = T + k * ZXY;
     const __amdgpu_buffer_rsrc_t rt = make_rsrc(Tk, (int)ZXY);
     const int z0 = p.zlo + zi;
@@ -275,18 +284,33 @@ __global__ __launch_bounds__(256) void cone_kernel(const float *__restrict__ mu,
         const bool wr = act & ((uint32_t)z < (uint32_t)Z) & ((uint32_t)y < (uint32_t)Y);
         const float tprev = p.accumulate ? load_f32(rt, wr ? base * 4u : kConeOOB) : 0.0f;
         float s[4] = {0.0f, 0.0f, 0.0f, 0.0f};
+        __asm__ volatile("" ::: "memory");  // re-read the tap table at every step
         if (zx_in && __all(yin || yout)) {
             const uint32_t vb = yin ? base * 4u : kConeOOB;
 #pragma unroll
-            for (int i = 0; i < NT; ++i) s[i & 3] = fmaf(p.w[i], load_f32(rs, vb + (uint32_t)p.off[i]), s[i & 3]);
+            for (int i = 0; i < NT; ++i) {
+                const int4 tp = tab[i];
+                s[i & 3] = fmaf(__int_as_float(tp.y), load_f32(rs, vb + (uint32_t)tp.x), s[i & 3]);
+            }
+        } else if (zx_in) {
+            // Some lane's stencil crosses a y face (the taps are inside in z and x):
+            // check the column of every tap, lane by lane.
+            const uint32_t b4 = base * 4u;
+#pragma unroll
+            for (int i = 0; i < NT; ++i) {
+                const int4 tp = tab[i];
+                const bool ok = act & ((uint32_t)(y + tp.z) < (uint32_t)Y);
+                s[i & 3] = fmaf(__int_as_float(tp.y), load_f32(rs, ok ? b4 + (uint32_t)tp.x : kConeOOB), s[i & 3]);
+            }
         } else {
 #pragma unroll
             for (int i = 0; i < NT; ++i) {
-                const int zz = z + p.dz[i], xx = x + p.t[i], yy = y + p.dy[i];
+                const int4 tp = tab[i];
+                const int zz = z + (int16_t)(tp.w & 0xffff), xx = x + (tp.w >> 16), yy = y + tp.z;
                 const bool ok = act & ((uint32_t)zz < (uint32_t)Z) & ((uint32_t)xx < (uint32_t)X) &
                                 ((uint32_t)yy < (uint32_t)Y);
-                const uint32_t v = ok ? base * 4u + (uint32_t)p.off[i] : kConeOOB;
-                s[i & 3] = fmaf(p.w[i], load_f32(rs, v), s[i & 3]);
+                const uint32_t v = ok ? base * 4u + (uint32_t)tp.x : kConeOOB;
+                s[i & 3] = fmaf(__int_as_float(tp.y), load_f32(rs, v), s[i & 3]);
             }
         }
         J += (s[0] + s[1]) + (s[2] + s[3]);
