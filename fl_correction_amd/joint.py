@@ -1,0 +1,113 @@
+"""Reconstruction through the attenuated model: the self-consistent MLEM loop
+and gradient-based joint reconstruction.
+
+Self-consistent MLEM (``self_consistent_mlem``): the attenuation maps are
+recomputed from the current estimate every k iterations and each emission line
+runs k MLEM iterations on them (the outer loop of the reference's demo,
+demo.py:244-336, with the estimate itself as the concentrations).
+
+Gradient-based joint reconstruction (``joint_reconstruct``):
+
+All element maps C (K, B, H, W) are fitted to the measured projections
+I (K, A, B, W) through ``XRFModel``: the attenuation is recomputed from C at
+every evaluation, so the gradient includes its dependence on C (the term the
+reference's autograd wrapper, FL/cuda_lib/autograd.py:40-74, leaves out).
+Loss: the reference's Poisson negative log-likelihood ``poisson_loss_image``
+(FL/ml_lib/ml_loss.py:6-8), mean(P - I*log P) with P = max(Pf, eps).
+Optimiser: Adam with a projection onto C >= 0 after every step.  The start is
+an MLEM estimate (``mlem_start``): per emission line, ``MLEMSolver`` on the
+attenuation of the initial estimate.
+
+Every model evaluation runs the library's kernels (forward, vjp); the loss and
+the Adam update are elementwise torch operations on the device.
+"""
+from __future__ import annotations
+
+import torch
+
+from .mlem import MLEMSolver
+from .model import XRFModel, XRFModelFunction
+
+
+def poisson_loss(Pf, I, eps=1e-6):
+    """mean(P - I*log P), P = max(Pf, eps)   (FL/ml_lib/ml_loss.py:6-8)."""
+    P = Pf.clamp_min(eps)
+    return (P - I * torch.log(P)).mean()
+
+
+def data_residual(Pf, I):
+    """Relative L2 misfit ||Pf - I|| / ||I|| of the projections."""
+    return float(torch.linalg.vector_norm(Pf - I) / torch.linalg.vector_norm(I))
+
+
+def mlem_start(model: XRFModel, I, C_init, n_iter=10, beta=0.0, delta=0.01):
+    """n_iter MLEM iterations per emission line, on the attenuation of C_init.
+
+    I (K, A, B, W), C_init (K, B, H, W); returns a new (K, B, H, W) tensor."""
+    C_init = C_init.to(model.device, torch.float32).contiguous()
+    I = I.to(model.device, torch.float32).contiguous()
+    atten = model.attenuation(C_init)  # (K, A, B, H, W)
+    out = torch.empty_like(C_init)
+    for e in range(model.K):
+        em = model.em_T[e][:, None].contiguous()  # (A, 1): one reference per line
+        solver = MLEMSolver(atten[e], em, model.theta, I[e].contiguous(), beta=beta, delta=delta)
+        out[e] = solver.run(C_init[e:e + 1], n_iter)[0]
+    del atten
+    return out
+
+
+def self_consistent_mlem(model: XRFModel, I, C_init, n_iter=50, recompute_every=5, beta=0.0, delta=0.01,
+                         callback=None):
+    """n_iter MLEM iterations per emission line, the attenuation recomputed from
+    the current estimate before every block of `recompute_every` iterations.
+
+    I (K, A, B, W), C_init (K, B, H, W); returns a new (K, B, H, W) tensor.
+    callback(iteration, C) is called after every block if given."""
+    if recompute_every < 1:
+        raise ValueError("recompute_every must be >= 1")
+    C = C_init.to(model.device, torch.float32).clone().contiguous()
+    I = I.to(model.device, torch.float32).contiguous()
+    done = 0
+    while done < n_iter:
+        k = min(int(recompute_every), int(n_iter) - done)
+        atten = model.attenuation(C)  # (K, A, B, H, W) for the current estimate
+        for e in range(model.K):
+            em = model.em_T[e][:, None].contiguous()
+            solver = MLEMSolver(atten[e], em, model.theta, I[e].contiguous(), beta=beta, delta=delta)
+            C[e] = solver.run(C[e:e + 1], k)[0]
+            del solver
+        del atten
+        done += k
+        if callback is not None:
+            callback(done, C)
+    return C
+
+
+def joint_reconstruct(model: XRFModel, I, C0, n_steps=100, lr=None, eps=1e-6, callback=None):
+    """Adam on all element maps with a non-negativity projection.
+
+    I (K, A, B, W) measured projections, C0 (K, B, H, W) start.  lr defaults to
+    2% of the start's mean value.  Returns (C, history) with history a list of
+    (step, loss, data residual) before each update and after the last one.
+    callback(step, C, loss) is called after every step if given."""
+    I = I.to(model.device, torch.float32).contiguous()
+    C = C0.to(model.device, torch.float32).clone().contiguous().requires_grad_(True)
+    if lr is None:
+        lr = 0.02 * float(C0.mean().clamp_min(1e-6))
+    opt = torch.optim.Adam([C], lr=lr)
+    history = []
+    for step in range(int(n_steps) + 1):
+        opt.zero_grad(set_to_none=True)
+        Pf = XRFModelFunction.apply(C, model)
+        loss = poisson_loss(Pf, I, eps)
+        history.append((step, float(loss.detach()), data_residual(Pf.detach(), I)))
+        if step == n_steps:
+            break
+        loss.backward()
+        opt.step()
+        with torch.no_grad():
+            C.clamp_(min=0.0)
+        if callback is not None:
+            callback(step, C, float(loss.detach()))
+    model.release()
+    return C.detach(), history
