@@ -64,7 +64,8 @@ def main():
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus != 1 or world != 1:
         print("bench.py runs on one GPU only: this build has no multi-GPU path "
-              "(the design for sharding slices or angles is described in DESIGN.md).", file=sys.stderr)
+              "(slices shard with no communication while the attenuation is fixed; that path is not built).",
+              file=sys.stderr)
         sys.exit(2)
 
     import torch
@@ -207,6 +208,18 @@ def main():
 
     if not args.no_cpu_baseline:
         result["cpu_baseline"] = cpu_baseline(args)
+        if "model" in result:
+            # the iteration that recomputes the attenuation for every angle first
+            m = result["model"]
+            cpu_att = cpu_attenuation_sample(args)
+            gpu_ms = ms_per_step + A * m["attenuation_ms_per_angle"]
+            cpu_s = 1.0 / result["cpu_baseline"]["value"] + A * cpu_att["s_per_angle"]
+            result["iteration_with_attenuation"] = {
+                "gpu_ms": gpu_ms, "cpu_s": cpu_s, "speedup": cpu_s * 1e3 / gpu_ms,
+                "cpu_attenuation": cpu_att,
+                "note": "MLEM step plus attenuation maps of all lines at every angle (5x5 cone over 15 deg); "
+                        "GPU attenuation per angle from the model timings, CPU from the oracle's direct FP64 march "
+                        "on a slab, both scaled per angle"}
 
     print(json.dumps(result), flush=True)
 
@@ -368,6 +381,29 @@ def batch_timings(args):
         "tile_shapes": {k.split("|")[1]: v for k, v in tuner.table.items()},
         "rel_l2_batch_vs_sequential_first4": agree,
     }
+
+
+def cpu_attenuation_sample(args):
+    """The oracle's direct FP64 cone march (every voxel sums its rays' bilinear
+    samples over the full depth, the per-voxel form of the reference's CPU
+    attenuation, FL_correction_core.py:1498-1518) on a slab of 4 slices of one
+    line, all host cores; scaled to all slices and lines of one angle."""
+    ncores = len(os.sched_getaffinity(0))
+    os.environ.setdefault("OMP_NUM_THREADS", str(ncores))
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle  # the checker's CPU restatement, used here as the timed CPU port
+    from fl_correction_amd.detector_mask import Cone
+
+    N, K, Zs = args.size, args.elements, 4
+    cone = Cone(n=5, half_angle_deg=15.0, q=8)
+    mu = np.random.default_rng(5).uniform(0, 0.05, (1, Zs, N, N)).astype(np.float32)
+    t0 = time.perf_counter()
+    oracle.cone_march_f64(mu, cone.pz, cone.py, cone.qd, cone.weight.astype(np.float64))
+    dt = time.perf_counter() - t0
+    scale = (N / Zs) * K
+    log(f"cpu attenuation sample: {dt:.2f}s for {Zs} slices of one line")
+    return {"s_per_angle": dt * scale, "cores": ncores,
+            "sample": f"{Zs} of {N} slices, 1 of {K} lines, one angle at {N}^3: {dt:.2f}s, scaled x{scale:g}"}
 
 
 def cpu_baseline(args):

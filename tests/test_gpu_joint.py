@@ -1,5 +1,5 @@
 """Gradient-based joint reconstruction through the attenuated model (Adam with a
-non-negativity projection from a 10-iteration MLEM start), judged by the data
+non-negativity projection from a 10-iteration MLEM start), measured by the data
 residual against the MLEM start.  Every model evaluation runs the HIP kernels."""
 import numpy as np
 import pytest
