@@ -397,13 +397,17 @@ def cpu_attenuation_sample(args):
     N, K, Zs = args.size, args.elements, 4
     cone = Cone(n=5, half_angle_deg=15.0, q=8)
     mu = np.random.default_rng(5).uniform(0, 0.05, (1, Zs, N, N)).astype(np.float32)
-    t0 = time.perf_counter()
-    oracle.cone_march_f64(mu, cone.pz, cone.py, cone.qd, cone.weight.astype(np.float64))
-    dt = time.perf_counter() - t0
+    runs = []
+    for _ in range(3):
+        t0 = time.perf_counter()
+        oracle.cone_march_f64(mu, cone.pz, cone.py, cone.qd, cone.weight.astype(np.float64))
+        runs.append(time.perf_counter() - t0)
+    dt = float(np.median(runs))
     scale = (N / Zs) * K
-    log(f"cpu attenuation sample: {dt:.2f}s for {Zs} slices of one line")
+    log(f"cpu attenuation sample: median {dt:.2f}s of {len(runs)} runs for {Zs} slices of one line")
     return {"s_per_angle": dt * scale, "cores": ncores,
-            "sample": f"{Zs} of {N} slices, 1 of {K} lines, one angle at {N}^3: {dt:.2f}s, scaled x{scale:g}"}
+            "sample": f"{Zs} of {N} slices, 1 of {K} lines, one angle at {N}^3: median of 3 runs {dt:.2f}s, "
+                      f"scaled x{scale:g}"}
 
 
 def cpu_baseline(args):
@@ -425,17 +429,20 @@ def cpu_baseline(args):
     em_np = np.full((ka, 1), 1e-4, np.float32)
     th_np = np.deg2rad(np.linspace(0, 180, ka, endpoint=False)).astype(np.float32)
     sens = oracle.backward_batch_f32(a_np, np.ones_like(i_np), em_np, th_np)  # untimed, as on the GPU
-    t0 = time.perf_counter()
-    Pf = oracle.forward_batch_f32(a_np, c_np, em_np, th_np)
-    ratio = i_np / np.maximum(Pf, np.float32(1e-6))
-    back = oracle.backward_batch_f32(a_np, ratio, em_np, th_np)
-    c_new = np.maximum(c_np * back / np.maximum(sens, np.float32(1e-6)), 0)
-    dt = time.perf_counter() - t0
-    del c_new
+    runs = []
+    for _ in range(3):
+        t0 = time.perf_counter()
+        Pf = oracle.forward_batch_f32(a_np, c_np, em_np, th_np)
+        ratio = i_np / np.maximum(Pf, np.float32(1e-6))
+        back = oracle.backward_batch_f32(a_np, ratio, em_np, th_np)
+        c_new = np.maximum(c_np * back / np.maximum(sens, np.float32(1e-6)), 0)
+        runs.append(time.perf_counter() - t0)
+        del c_new
+    dt = float(np.median(runs))
     scale = (args.angles / ka) * args.elements
-    log(f"cpu baseline: {dt:.2f}s for one line, {ka} angles, {ncores} threads")
+    log(f"cpu baseline: median {dt:.2f}s of 3 runs for one line, {ka} angles, {ncores} threads")
     return {"value": 1.0 / (dt * scale), "unit": "iter/s", "cores": ncores, "kind": "port",
-            "sample": f"one line, {ka} of {args.angles} angles at {N}^3: {dt:.2f}s on {ncores} threads, "
+            "sample": f"one line, {ka} of {args.angles} angles at {N}^3: median of 3 runs {dt:.2f}s on {ncores} threads, "
                       f"scaled x{scale:g} (lines are independent)"}
 
 

@@ -126,3 +126,24 @@ def test_autotuner_picks_and_caches(tmp_path):
     assert len(tu.timings) == len(FWD_ANGLES) * len(BWD_ROWS)
     again = Autotuner(path=str(path))
     assert again.shapes(30, 1, 33, 64) == (fa, br)  # same bucket, from the cache
+
+
+def test_batch_edge_sizes_and_iteration_counts():
+    """Minimum shapes (H = W = 2, one angle, one slice) beside a normal one;
+    n_iter = 0 leaves C unchanged, n_iter = 1 runs the odd-iteration path."""
+    shapes = [(1, 1, 2, 2), (2, 1, 3, 5), (3, 2, 70, 2), (1, 3, 2, 65), (5, 2, 33, 33)]
+    data = [_problem(*s, seed=300 + i) for i, s in enumerate(shapes)]
+    print(f"batch edge sizes {shapes}", flush=True)
+    run, probs = _batch(data)
+    before = [p.C.clone() for p in probs]
+    run.run(0, graph=True)
+    torch.cuda.synchronize()
+    for p, b in zip(probs, before):
+        assert torch.equal(p.C, b)
+    run.run(1, graph=True)
+    torch.cuda.synchronize()
+    for i, (d, p) in enumerate(zip(data, probs)):
+        a, e, t, I, C0 = d
+        ref = oracle.mlem_batch_f32(C0[None], a, e[:, None], t, I, 1, 0.0, 0.01)[0]
+        assert rel_l2(p.C.cpu().numpy(), ref) <= RTOL, shapes[i]
+    run.close()
