@@ -317,12 +317,12 @@ __device__ __forceinline__ int lanes_below(uint64_t mask)
 
 // Per-wave view of the back-projector's state for one tile.
 struct BwdTile {
-    int I0, J0, b;
+    int I0, J0, b, bi;         // bi: tile height
+    int edges;                 // image edges whose border cells lie in the tile (kEdge*)
     float lox, hix, loy, hiy;  // region of unclamped sample positions
     float tol;                 // rounding allowance of one sample coordinate
     float *lds;                // dynamic LDS: accumulators (cell-major, NR values per cell), sinks
     int cbase;                 // image cell (y0, x0) -> accumulator cell y0 * kBPitch + x0 + cbase
-    uint32_t *owner;           // election table (clamped samples)
     uint32_t *mask;            // row starts of the current window, two words per item
     int4 *rows;                // compacted rows of the current batch
     int lane, sink;            // sink: this lane's sink cell
@@ -338,9 +338,8 @@ struct BwdTile {
 // its samples lie >= 2 apart on one row or on rows >= 2 apart: >= 1.41 apart in
 // one coordinate, so their base cells differ, and a plain read-add-write, one
 // corner at a time, cannot lose an update (float LDS atomics are far slower on
-// this hardware).  Samples moved by the clamp (border tiles only) can share a
-// cell; they are added afterwards, one lane per cell per round, that lane
-// elected by an integer minimum.
+// this hardware).  Only unclamped samples are enumerated; the clamped ones are
+// gathered per border cell (bwd_edge).
 struct BwdCursor {
     int a;
     int cls, rb, hlo, hhi;
@@ -488,35 +487,24 @@ __device__ __forceinline__ void bwd_gather(const BwdArgs &p, const BwdTile &t, B
     for (int rr = 0; rr < NR; ++rr) g.ew[rr] = cur.ew[rr];
 }
 
-// Splat one item pair into the wave's accumulator.  BORDER: the tile touches
-// the image edge, so samples may be clamped (interior tiles have none and skip
-// the clamp entirely).
-template <int NR, bool BORDER>
-__device__ __forceinline__ void bwd_splat(const BwdArgs &p, const BwdTile &t, const BwdPair &g)
+// Splat one item pair into the wave's accumulator.  Every sample here is
+// unclamped (the clamped ones are summed per border cell by bwd_edge).
+template <int NR>
+__device__ __forceinline__ void bwd_splat(const BwdTile &t, const BwdPair &g)
 {
-    const float xmax = p.H - 1.001f, ymax = p.W - 1.001f;
     float *lds = t.lds;
     int cell[kBUnroll][2];
     float wq[kBUnroll][2][4];
-    bool fast[kBUnroll][2], slow[kBUnroll][2];
 #pragma unroll
     for (int u = 0; u < kBUnroll; ++u)
 #pragma unroll
         for (int pt = 0; pt < 2; ++pt) {
-            float Xc = g.X[u][pt], Yc = g.Y[u][pt];
-            bool cl = false;
-            if (BORDER) {
-                Xc = clampf(Xc, 0.f, xmax);
-                Yc = clampf(Yc, 0.f, ymax);
-                cl = Xc != g.X[u][pt] || Yc != g.Y[u][pt];
-            }
             // sample positions are >= 0 here, so (int) is floor and fract is the
             // exact remainder
+            const float Xc = g.X[u][pt], Yc = g.Y[u][pt];
             const int y0 = (int)Xc, x0 = (int)Yc;
             const float dy = __builtin_amdgcn_fractf(Xc), dx = __builtin_amdgcn_fractf(Yc);
-            cell[u][pt] = y0 * kBPitch + x0 + t.cbase;
-            fast[u][pt] = g.val[u][pt] && !cl;
-            slow[u][pt] = g.val[u][pt] && cl;
+            cell[u][pt] = g.val[u][pt] ? y0 * kBPitch + x0 + t.cbase : t.sink;
             // w*(1-dx)*(1-dy), w*dx*(1-dy), w*(1-dx)*dy, w*dx*dy  (.cu:434-443),
             // formed as differences: w*(1-dx) = w - w*dx
             const float v = g.av[u][pt] * g.rv[u][pt];
@@ -532,7 +520,7 @@ __device__ __forceinline__ void bwd_splat(const BwdArgs &p, const BwdTile &t, co
     for (int pt = 0; pt < 2; ++pt) {
         float *pa[kBUnroll];
 #pragma unroll
-        for (int u = 0; u < kBUnroll; ++u) pa[u] = lds + (fast[u][pt] ? cell[u][pt] : t.sink) * NR;
+        for (int u = 0; u < kBUnroll; ++u) pa[u] = lds + cell[u][pt] * NR;
 #pragma unroll
         for (int k = 0; k < 4; ++k) {
             if constexpr (NR == 1) {
@@ -556,31 +544,289 @@ __device__ __forceinline__ void bwd_splat(const BwdArgs &p, const BwdTile &t, co
             __asm__ volatile("" ::: "memory");
         }
     }
-    if (BORDER) {
+}
+
+// Clamped samples.  The reference moves a sample position outside
+// [0, H-1.001] x [0, W-1.001] onto the border before splatting it
+// (.cu:421-424), so every such sample lands on a border base cell: row 0 (top,
+// X < 0), row H-2 (bottom, X > H-1.001), column 0 (left, Y < 0, X inside) or
+// column W-2 (right, Y > W-1.001, X inside); a sample clamped in both
+// coordinates counts as top or bottom.  Many of them share a cell within one
+// splat instruction, so they are gathered per border cell instead: in a tile
+// that touches an edge, lane k owns one of the tile's border base cells and,
+// per angle, walks the strip of detector points that clamp into it, testing
+// each candidate with the splat's own arithmetic, and sums its weights in
+// registers; the four corner sums are then added to the accumulator one
+// corner at a time (the lanes' cells differ, so no two lanes collide).
+enum : int { kEdgeTop = 1, kEdgeBottom = 2, kEdgeLeft = 4, kEdgeRight = 8 };
+
+// A buffer offset past every slice (H*W*4 < 2^31 bytes): the load returns 0.
+constexpr uint32_t kOOB = 0x80000000u;
+
+// One bound on the minor coordinate n as a linear function of the major one:
+// n >= u*m + v (lower) or n <= u*m + v (upper); inactive bounds are +-1e30.
+struct LinBound {
+    float u, v;
+};
+
+// Half-plane a*m + b*n + g >= 0 as a bound on n.  A half-plane that hardly
+// depends on n (|b| below 1/16) becomes a condition on m alone, relaxed by the
+// largest |b*n| over the box so that it never loses a point.
+__device__ __forceinline__ void half_plane(float a, float b, float g, float nabs, LinBound &lo, LinBound &hi,
+                                           float &mlo, float &mhi)
+{
+    constexpr float kBig = 1.0e30f;
+    lo = {0.0f, -kBig};
+    hi = {0.0f, kBig};
+    if (b >= 0.0625f) {
+        const float r = 1.0f / b;
+        lo = {-a * r, -g * r};
+    } else if (b <= -0.0625f) {
+        const float r = -1.0f / b;
+        hi = {a * r, g * r};
+    } else {
+        const float gg = g + fabsf(b) * nabs;
+        if (a > 0.0f)
+            mlo = fmaxf(mlo, -gg / a);
+        else if (a < 0.0f)
+            mhi = fminf(mhi, gg / -a);
+        else if (gg < 0.0f)
+            mhi = mlo - 1.0f;  // empty
+    }
+}
+
+// Narrow [mlo, mhi] to where lower bound L can lie below upper bound U.
+__device__ __forceinline__ void pair_range(const LinBound &L, const LinBound &U, float &mlo, float &mhi)
+{
+    const float du = L.u - U.u, dv = U.v - L.v;  // need du*m <= dv
+    if (du > 0.0f)
+        mhi = fminf(mhi, dv / du);
+    else if (du < 0.0f)
+        mlo = fmaxf(mlo, dv / du);
+    else if (dv < 0.0f)
+        mhi = mlo - 1.0f;
+}
+
+// One edge candidate pair: detector points (m, n) and (m, n + 1) of the walk.
+struct EdgePair {
+    float av[2], rv[2];  // atten and ratio (0 where the point is not taken)
+    float fr[2];         // fraction along the edge: dx (top, bottom) or dy (left, right)
+};
+
+template <int NR, bool RATIO, int EDGE>
+__device__ __forceinline__ void bwd_edge(const BwdArgs &p, const BwdTile &t, const BwdCursor &cur)
+{
+    constexpr bool xedge = EDGE == kEdgeTop || EDGE == kEdgeBottom;  // clamped in X (rows 0 / H-2)
+    constexpr bool low_edge = EDGE == kEdgeTop || EDGE == kEdgeLeft;
+    const float cx = (p.W - 1) * 0.5f, cy = (p.H - 1) * 0.5f;
+    const float xmax = p.H - 1.001f, ymax = p.W - 1.001f;
+    const float c = cur.c, s = cur.s;
+    // the lane's border base cell: column k of row 0 / H-2, or row k of column 0 / W-2
+    const int k = (xedge ? t.J0 : t.I0) - 1 + t.lane;
+    const int kmax = xedge ? p.W - 2 : p.H - 2;
+    const int nlanes = xedge ? kBJ + 1 : t.bi + 1;
+    const bool active = t.lane < nlanes && k >= 0 && k <= kmax;
+    // Walk the detector axis along the strip (major m), the other one per step
+    // (minor n): the strip coordinate P (Y for top/bottom, X for left/right)
+    // changes by at least 0.7 per minor step, so a step holds at most 2 points.
+    //   X - cy = c*hc + s*qc,  Y - cx = -s*hc + c*qc   (hc = h - cy, qc = q - cx)
+    const bool major_h = xedge ? fabsf(c) >= fabsf(s) : fabsf(c) < fabsf(s);
+    const float cm = major_h ? cy : cx, cn = major_h ? cx : cy;
+    const int Mmax = major_h ? p.H - 1 : p.W - 1, Nmax = major_h ? p.W - 1 : p.H - 1;
+    const float aX = major_h ? c : s, bX = major_h ? s : c;
+    const float aY = major_h ? -s : c, bY = major_h ? c : -s;
+    const float aP = xedge ? aY : aX, bP = xedge ? bY : bX, P0 = xedge ? cx : cy;
+    const float aQ = xedge ? aX : aY, bQ = xedge ? bX : bY, Q0 = xedge ? cy : cx;
+    constexpr float kBig = 1.0e30f;
+    // Every bound below is relaxed by tol (coordinate units), far more than the
+    // rounding of the splat's arithmetic: the walk visits a superset of the
+    // strip, and each point is then tested exactly.
+    const float tol = 2.0f * t.tol + 0.01f;
+    // strip kk <= P < kk+1 (top/bottom: column 0 also takes Y < 0, column W-2
+    // also Y > W-1.001): n between slo(mc) and shi(mc)
+    const float rP = 1.0f / bP;
+    const auto strip = [&](int kk, LinBound &lo, LinBound &hi) {
+        const float plo = (xedge && kk == 0) ? -kBig : (float)kk - tol;
+        const float phi = (xedge && kk == kmax) ? kBig : (float)(kk + 1) + tol;
+        const float vA = (plo - P0) * rP, vB = (phi - P0) * rP;
+        lo = {-aP * rP, fminf(vA, vB)};
+        hi = {-aP * rP, fmaxf(vA, vB)};
+    };
+    // depth beyond the edge, dA*mc + dB*nc + dG >= 0: Q < 0 (top, left) or
+    // Q > H-1.001 / W-1.001 (bottom, right)
+    const float qlim = low_edge ? 0.0f : (xedge ? xmax : ymax);
+    const float dA = low_edge ? -aQ : aQ, dB = low_edge ? -bQ : bQ;
+    const float dG = low_edge ? tol - Q0 : Q0 - qlim + tol;
+    float mlo = -cm, mhi = (float)Mmax - cm;
+    LinBound dlo, dhi;
+    half_plane(dA, dB, dG, fmaxf(cn, (float)Nmax - cn), dlo, dhi, mlo, mhi);
+    // steps m0..m1 of this lane's own strip
+    int m0, m1;
+    {
+        LinBound slo, shi;
+        strip(k, slo, shi);
+        const LinBound L[3] = {{0.0f, -cn}, slo, dlo}, U[3] = {{0.0f, (float)Nmax - cn}, shi, dhi};
 #pragma unroll
-        for (int pt = 0; pt < 2; ++pt)
+        for (int i = 0; i < 3; ++i)
 #pragma unroll
-            for (int u = 0; u < kBUnroll; ++u) {
-                bool pend = slow[u][pt];
-                const int slot = cell[u][pt] & (kBSlots - 1);
-                while (__ballot(pend)) {
-                    if (pend) atomicMin(&t.owner[slot], (uint32_t)t.lane);
-                    __asm__ volatile("" ::: "memory");
-                    const bool won = pend && t.owner[slot] == (uint32_t)t.lane;
-                    if (won) {
-                        float *d = lds + cell[u][pt] * NR;
+            for (int j = 0; j < 3; ++j) pair_range(L[i], U[j], mlo, mhi);
+        m0 = (int)fmaxf(ceilf(mlo + cm - 0.01f), 0.0f);
+        m1 = active ? (int)fminf(floorf(mhi + cm + 0.01f), (float)Mmax) : -1;
+    }
+    const int len = max(m1 - m0 + 1, 0);
+
+    // Balance: the strips are cut into chunks of at most Lc steps, at most 64
+    // chunks in all, and lane j walks chunk j.  Chunk table: the non-empty
+    // strips compacted (cell, m0, m1, first chunk); the start bits of the strips'
+    // first chunks give each lane its strip.
+    const int total = lane_value(wave_inclusive_scan(len), 63);
+    if (total == 0) return;
+    const uint64_t nem = __ballot(len > 0);
+    const int spare = max(64 - __popcll(nem), 1);
+    const int Lc = max((total + spare - 1) / spare, 1);
+    const int nch = (len + Lc - 1) / Lc;
+    const int cP = wave_inclusive_scan(nch) - nch;  // first chunk of this lane's strip
+    const int nchunks = lane_value(cP + nch, 63);
+    int4 *tab = t.rows;
+    if (t.lane < 2) t.mask[t.lane] = 0u;
+    __asm__ volatile("" ::: "memory");
+    if (len > 0) {
+        tab[lanes_below(nem)] = make_int4(k, m0, m1, cP);
+        atomicOr(&t.mask[cP >> 5], 1u << (cP & 31));
+    }
+    __asm__ volatile("" ::: "memory");
+    const uint2 mw = *reinterpret_cast<const uint2 *>(t.mask);
+    const uint64_t starts = (uint64_t)mw.x | ((uint64_t)mw.y << 32);
+    const int ord = __popcll(starts & (t.lane == 63 ? ~0ull : (2ull << t.lane) - 1ull)) - 1;
+    const int4 ce = tab[max(ord, 0)];
+    const int ck = ce.x;                              // the chunk's cell
+    int m = ce.y + (t.lane - ce.w) * Lc;              // its first step
+    const int mend = t.lane < nchunks ? min(m + Lc - 1, ce.z) : -1;
+    LinBound slo, shi;
+    strip(ck, slo, shi);
+
+    // minor range of step mm
+    const auto nrange = [&](int mm, int &na, int &nb) {
+        const float mc = (float)mm - cm;
+        const float lo = fmaxf(fmaf(slo.u, mc, slo.v), fmaf(dlo.u, mc, dlo.v));
+        const float hi = fminf(fmaf(shi.u, mc, shi.v), fmaf(dhi.u, mc, dhi.v));
+        na = (int)fmaxf(ceilf(lo + cn - 0.01f), 0.0f);
+        nb = (int)fminf(floorf(hi + cn + 0.01f), (float)Nmax);
+    };
+    // candidates (mm, nn) and (mm, nn+1): the splat's arithmetic (bwd_gather),
+    // then the exact test; the loads of points not taken read 0
+    const auto gather = [&](int mm, int nn, int nb, EdgePair &e) {
 #pragma unroll
-                        for (int k = 0; k < 4; ++k) {
-#pragma unroll
-                            for (int rr = 0; rr < NR; ++rr) d[offs[k] + rr] = fmaf(g.ew[rr], wq[u][pt][k], d[offs[k] + rr]);
-                            __asm__ volatile("" ::: "memory");
-                        }
-                        t.owner[slot] = 0xffffffffu;
-                    }
-                    __asm__ volatile("" ::: "memory");
-                    pend = pend && !won;
-                }
+        for (int i = 0; i < 2; ++i) {
+            const int h = major_h ? mm : nn + i, q = major_h ? nn + i : mm;
+            const float hc = (float)h - cy, qc = (float)q - cx;
+            const float X = fmaf(s, qc, fmaf(c, hc, cy)), Y = fmaf(c, qc, fmaf(-s, hc, cx));
+            bool in;
+            if constexpr (xedge) {
+                const float Yc = clampf(Y, 0.f, ymax);
+                in = (low_edge ? X < 0.0f : X > xmax) && (int)Yc == ck;
+                e.fr[i] = __builtin_amdgcn_fractf(Yc);
+            } else {
+                in = X >= 0.0f && X <= xmax && (low_edge ? Y < 0.0f : Y > ymax) && (int)X == ck;
+                e.fr[i] = __builtin_amdgcn_fractf(X);
             }
+            in = in && nn + i <= nb;
+            e.av[i] = load_f32(cur.ra, in ? (uint32_t)(h * p.W + q) * 4u : kOOB);
+            e.rv[i] = RATIO ? load_f32(cur.rr, in ? (uint32_t)q * 4u : kOOB) : 1.0f;
+        }
+    };
+
+    // Sums over the chunk: A[0] of ew*w*(1-f), A[1] of ew*w*f, f the fraction
+    // along the edge; the other coordinate's fraction is the same for every
+    // point of an edge (0 at top/left, that of H-1.001 / W-1.001 at bottom/right)
+    // and is applied when the sums are added.
+    float A[2][NR];
+#pragma unroll
+    for (int j = 0; j < 2; ++j)
+#pragma unroll
+        for (int rr = 0; rr < NR; ++rr) A[j][rr] = 0.0f;
+    const auto add = [&](const EdgePair &e) {
+#pragma unroll
+        for (int i = 0; i < 2; ++i) {
+            // w*(1-f), w*f: the splat's differences (.cu:434-443)
+            const float v = e.av[i] * e.rv[i];
+            const float w1 = v * e.fr[i], w0 = v - w1;
+#pragma unroll
+            for (int rr = 0; rr < NR; ++rr) {
+                A[0][rr] = fmaf(cur.ew[rr], w0, A[0][rr]);
+                A[1][rr] = fmaf(cur.ew[rr], w1, A[1][rr]);
+            }
+        }
+    };
+    int n, nb;
+    if (m <= mend) nrange(m, n, nb);
+    // skip empty steps
+    const auto advance = [&]() {
+        while (m <= mend && n > nb) {
+            ++m;
+            if (m <= mend) nrange(m, n, nb);
+        }
+    };
+    // two pair buffers used in turn, so no register copy waits on a load in flight
+    advance();
+    EdgePair e0, e1;
+    if (m <= mend) gather(m, n, nb, e0);
+    while (m <= mend) {
+        n += 2;
+        advance();
+        const bool more = m <= mend;
+        if (more) gather(m, n, nb, e1);
+        add(e0);
+        if (!more) break;
+        n += 2;
+        advance();
+        const bool more2 = m <= mend;
+        if (more2) gather(m, n, nb, e0);
+        add(e1);
+        if (!more2) break;
+    }
+
+    // Chunk sums in the table's place (it has been read), then each cell's lane
+    // adds its chunks in order.
+    float *part = reinterpret_cast<float *>(t.rows);
+    __asm__ volatile("" ::: "memory");
+#pragma unroll
+    for (int j = 0; j < 2; ++j)
+#pragma unroll
+        for (int rr = 0; rr < NR; ++rr) part[(t.lane * 2 + j) * NR + rr] = A[j][rr];
+    __asm__ volatile("" ::: "memory");
+    float S[2][NR];
+#pragma unroll
+    for (int j = 0; j < 2; ++j)
+#pragma unroll
+        for (int rr = 0; rr < NR; ++rr) S[j][rr] = 0.0f;
+    for (int i = 0; i < nch; ++i)
+#pragma unroll
+        for (int j = 0; j < 2; ++j)
+#pragma unroll
+            for (int rr = 0; rr < NR; ++rr) S[j][rr] += part[((cP + i) * 2 + j) * NR + rr];
+    __asm__ volatile("" ::: "memory");
+
+    // Add to the accumulator, one corner at a time (corner j of base cell
+    // (y0, x0) is (y0 + j/2, x0 + j%2); the lanes' base cells differ).
+    const int y0 = EDGE == kEdgeTop ? 0 : EDGE == kEdgeBottom ? p.H - 2 : k;
+    const int x0 = EDGE == kEdgeLeft ? 0 : EDGE == kEdgeRight ? p.W - 2 : k;
+    float *pa = t.lds + (len > 0 ? y0 * kBPitch + x0 + t.cbase : t.sink) * NR;
+    constexpr int offs[4] = {0, NR, kBPitch * NR, (kBPitch + 1) * NR};
+    // the fixed fraction: dy at the bottom edge, dx at the right edge
+    const float D = __builtin_amdgcn_fractf(EDGE == kEdgeBottom ? xmax : ymax);
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+        // along-edge index ja and across-edge index jb of corner j
+        const int ja = xedge ? (j & 1) : (j >> 1), jb = xedge ? (j >> 1) : (j & 1);
+        if (low_edge && jb == 1) continue;  // the fixed fraction is 0
+#pragma unroll
+        for (int rr = 0; rr < NR; ++rr) {
+            const float a = S[ja][rr];
+            const float part_j = low_edge ? a : (jb == 0 ? a - a * D : a * D);
+            pa[offs[j] + rr] += part_j;
+        }
+        __asm__ volatile("" ::: "memory");
     }
 }
 
@@ -588,13 +834,18 @@ __device__ __forceinline__ void bwd_splat(const BwdArgs &p, const BwdTile &t, co
 // classes, their batches and the batches' windows.  Inside a window the loads
 // of the next item pair are issued before the current pair is splatted (two
 // pair buffers, used in turn, so no register copy waits on a load in flight).
-// RATIO: ratio is given (false: ratio = 1, the sensitivity).
-template <int NR, bool BORDER, bool RATIO>
+// RATIO: ratio is given (false: ratio = 1, the sensitivity).  Tiles on the
+// image border also gather their border cells' clamped samples per angle.
+template <int NR, bool RATIO>
 __device__ __forceinline__ void bwd_stream(const BwdArgs &p, const BwdTile &t, int wave)
 {
     BwdCursor cur;
     for (cur.a = wave; cur.a < p.A; cur.a += kBWaves) {
         bwd_begin_angle<NR>(p, t, cur);
+        if (t.edges & kEdgeTop) bwd_edge<NR, RATIO, kEdgeTop>(p, t, cur);
+        if (t.edges & kEdgeBottom) bwd_edge<NR, RATIO, kEdgeBottom>(p, t, cur);
+        if (t.edges & kEdgeLeft) bwd_edge<NR, RATIO, kEdgeLeft>(p, t, cur);
+        if (t.edges & kEdgeRight) bwd_edge<NR, RATIO, kEdgeRight>(p, t, cur);
         for (cur.cls = 0; cur.cls < 2; ++cur.cls) {
             for (cur.rb = cur.hlo + cur.cls; cur.rb <= cur.hhi; cur.rb += 2 * kBRows) {
                 bwd_begin_batch(p, t, cur);
@@ -606,12 +857,12 @@ __device__ __forceinline__ void bwd_stream(const BwdArgs &p, const BwdTile &t, i
                         cur.n += kBUnroll;
                         const bool more = cur.n < cur.nit;
                         if (more) bwd_gather<NR, RATIO>(p, t, cur, g1);
-                        bwd_splat<NR, BORDER>(p, t, g0);
+                        bwd_splat<NR>(t, g0);
                         if (!more) break;
                         cur.n += kBUnroll;
                         const bool more2 = cur.n < cur.nit;
                         if (more2) bwd_gather<NR, RATIO>(p, t, cur, g0);
-                        bwd_splat<NR, BORDER>(p, t, g1);
+                        bwd_splat<NR>(t, g1);
                         if (!more2) break;
                     }
                 }
@@ -623,9 +874,8 @@ __device__ __forceinline__ void bwd_stream(const BwdArgs &p, const BwdTile &t, i
 // Per-wave tables of the back-projector in static LDS.  One instance per
 // kernel: every tile height a kernel instantiates shares it.
 struct BwdTables {
-    uint32_t owner[kBWaves][kBSlots];                  // election table (clamped samples)
     __align__(16) uint32_t mask[kBWaves][kBItems * 2];  // row starts of the current window
-    int4 row[kBWaves][kBRows];                         // compacted rows of the current batch
+    int4 row[kBWaves][64];  // compacted rows of the current batch (kBRows); edge chunk table, partial sums
 };
 
 __device__ __forceinline__ BwdTables &bwd_tables()
@@ -655,19 +905,26 @@ __device__ __forceinline__ void bwd_tile(const BwdArgs &p, uint32_t id)
     t.I0 = ti * BI;
     t.J0 = tj * kBJ;
     t.b = b;
-    constexpr float BIG = 1.0e6f;
-    // Unclamped sample positions whose clamped base cell lies in
-    // [I0-1, I0+BI-1] x [J0-1, J0+kBJ-1]: open on a side where the clamp folds
-    // everything beyond onto the tile.
-    t.lox = t.I0 <= 1 ? -BIG : (float)(t.I0 - 1);
-    t.hix = t.I0 + BI >= p.H - 1 ? BIG : (float)(t.I0 + BI);
-    t.loy = t.J0 <= 1 ? -BIG : (float)(t.J0 - 1);
-    t.hiy = t.J0 + kBJ >= p.W - 1 ? BIG : (float)(t.J0 + kBJ);
+    t.bi = BI;
+    // Unclamped sample positions (0 <= X <= H-1.001, 0 <= Y <= W-1.001) whose
+    // base cell lies in [I0-1, I0+BI-1] x [J0-1, J0+kBJ-1]; the upper limits are
+    // the floats just above H-1.001 and W-1.001 (X < that iff X <= H-1.001).
+    const float xup = __int_as_float(__float_as_int(p.H - 1.001f) + 1);
+    const float yup = __int_as_float(__float_as_int(p.W - 1.001f) + 1);
+    t.lox = (float)max(t.I0 - 1, 0);
+    t.hix = fminf((float)(t.I0 + BI), xup);
+    t.loy = (float)max(t.J0 - 1, 0);
+    t.hiy = fminf((float)(t.J0 + kBJ), yup);
+    // Border base cells in the tile: row 0 / H-2, column 0 / W-2.
+    const auto holds = [](int lo, int hi, int v) { return lo <= v && v <= hi; };
+    t.edges = (holds(t.I0 - 1, t.I0 + BI - 1, 0) ? kEdgeTop : 0) |
+              (holds(t.I0 - 1, t.I0 + BI - 1, p.H - 2) ? kEdgeBottom : 0) |
+              (holds(t.J0 - 1, t.J0 + kBJ - 1, 0) ? kEdgeLeft : 0) |
+              (holds(t.J0 - 1, t.J0 + kBJ - 1, p.W - 2) ? kEdgeRight : 0);
     // Largest rounding error of one sample coordinate near an active bound.
     t.tol = 2.4e-7f * (float)(2 * max(p.H, p.W) + 4);
     t.lds = lds;
     t.cbase = wave * BACC + (1 - t.I0) * kBPitch + 1 - t.J0;
-    t.owner = tab.owner[wave];
     t.mask = tab.mask[wave];
     t.rows = tab.row[wave];
     t.lane = lane;
@@ -675,21 +932,12 @@ __device__ __forceinline__ void bwd_tile(const BwdArgs &p, uint32_t id)
     // read-modify-write there, each lane its own NR words.
     t.sink = kBWaves * BACC + lane;
     for (int k = lane; k < nacc; k += 64) lds[wave * nacc + k] = 0.0f;
-    for (int k = lane; k < kBSlots; k += 64) t.owner[k] = 0xffffffffu;
-    const bool border = t.lox < 0.0f || t.hix > (float)p.H || t.loy < 0.0f || t.hiy > (float)p.W;
 
     // Wave w takes angles w, w+4, ... into its own accumulator.
-    if (p.ratio) {
-        if (border)
-            bwd_stream<NR, true, true>(p, t, wave);
-        else
-            bwd_stream<NR, false, true>(p, t, wave);
-    } else {
-        if (border)
-            bwd_stream<NR, true, false>(p, t, wave);
-        else
-            bwd_stream<NR, false, false>(p, t, wave);
-    }
+    if (p.ratio)
+        bwd_stream<NR, true>(p, t, wave);
+    else
+        bwd_stream<NR, false>(p, t, wave);
     __syncthreads();
 
     // Sum the four private accumulators in a fixed order and write the tile (lane = column).

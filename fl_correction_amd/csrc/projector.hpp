@@ -52,7 +52,6 @@ constexpr int kBSink = 136;           // sink cells: lane + corner offset <= 63 
 constexpr int kBItems = 32;           // items of 128 samples per enumeration window
 constexpr int kBWin = 128 * kBItems;  // samples per window
 constexpr int kBUnroll = 2;           // items splatted together
-constexpr int kBSlots = 64;           // election table for clamped samples (hash of the cell)
 constexpr int kBRows = 48;            // detector rows per batch
 static_assert(kBItems * 2 == 64, "one mask word per lane");
 static_assert(kBItems % kBUnroll == 0, "items come in whole unroll groups");
