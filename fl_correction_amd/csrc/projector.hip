@@ -19,15 +19,14 @@
 // ratio I / max(Pf, 1e-6).  No atomics, no partial-sum buffer: deterministic.
 //
 // Back-projection: one workgroup owns a 31x63 image tile of one slice (the
-// clamped samples that touch it fill a 32x64 region); each of its 4 waves keeps
-// a private LDS accumulator (tile plus a one-voxel halo).  For every angle the
-// waves take turns over the detector rows of the tile's preimage: each row's
-// column interval comes from the rotated region's edges, and the row's points
-// are one coalesced segment of atten (lane = column; two rows per instruction
-// when rows are at most 32 long).  Four segments are loaded together, then the
-// bilinear weights of the points that land in the tile are splatted into the
-// wave's own accumulator.  No global atomics; the four accumulators are summed in
-// a fixed order at the end, which is also where the MLEM update is applied.
+// base cells of the samples that touch it fill a 32x64 region); each of its 4
+// waves keeps a private LDS accumulator (tile plus a one-voxel halo) and takes
+// the angles a = wave (mod 4).  Per angle a wave enumerates exactly the
+// detector points whose sample lands in the tile (rows of the tile's preimage,
+// each row's column interval) and splats their bilinear weights; the samples
+// the reference clamps onto the image border are gathered per border cell.  No
+// global atomics; the four accumulators are summed in a fixed order at the
+// end, which is also where the MLEM update is applied.
 #include "projector.hpp"
 
 #include <algorithm>
@@ -143,14 +142,15 @@ __device__ __forceinline__ void fwd_strip_tile(const FwdArgs &p, uint32_t t)
         }
         int gi = 0;
         while (gi < na) {
-            // Longest run of angles gi..gj-1 whose union footprint fits the budget.
+            // Longest run of angles gi..gj-1 whose union footprint fits the budget
+            // and the staged width.
             int rlo = lane_value(mine.rlo, gi), rhi = lane_value(mine.rhi, gi);
             int clo = lane_value(mine.clo, gi), chi = lane_value(mine.chi, gi);
             int gj = gi + 1;
             while (gj < na) {
                 const int mrlo = min(rlo, lane_value(mine.rlo, gj)), mrhi = max(rhi, lane_value(mine.rhi, gj));
                 const int mclo = min(clo, lane_value(mine.clo, gj)), mchi = max(chi, lane_value(mine.chi, gj));
-                if ((mrhi - mrlo + 1) * ((mchi - mclo + 1) | 1) * NR > kFLds) break;
+                if ((mrhi - mrlo + 1) * ((mchi - mclo + 1) | 1) * NR > kFLds || mchi - mclo + 1 > kFStageCols) break;
                 rlo = mrlo;
                 rhi = mrhi;
                 clo = mclo;
@@ -166,8 +166,8 @@ __device__ __forceinline__ void fwd_strip_tile(const FwdArgs &p, uint32_t t)
             gj = uniform_int(gj);
 
             __syncthreads();  // everyone is done with the previous footprint
-            // Staging: wave w takes footprint rows w, w+4, ..., lanes the columns;
-            // eight loads in flight per lane.
+            // Staging: wave w takes footprint rows w, w+4, ..., lane l the columns
+            // l and l + 64 (cols <= kFStageCols); eight loads in flight per lane.
 #pragma unroll
             for (int rr = 0; rr < NR; ++rr) {
                 const float *Cs = p.C + ((int64_t)(p.r0 + rr) * p.B + b) * HW + (int64_t)rlo * p.W + clo;

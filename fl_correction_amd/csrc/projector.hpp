@@ -20,6 +20,10 @@ constexpr int kFLds = 11264;              // footprint floats staged per workgro
 // neighbour that is <= 74 rows and a pitch <= 75.
 constexpr int kFMaxFootprint = 74 * 75;
 constexpr int kFRefChunk = kFLds / kFMaxFootprint;  // references staged together (2)
+// Footprint columns the staging covers (two per lane); a union of several
+// angles' footprints may not be wider.  One angle's is at most 75.
+constexpr int kFStageCols = 2 * 64;
+static_assert(75 <= kFStageCols, "one angle's footprint must fit the staged width");
 
 enum FwdMode { kFwdWrite = 0, kFwdAccumulate = 1, kFwdRatio = 2, kFwdAccumulateRatio = 3 };
 

@@ -147,3 +147,25 @@ def test_batch_edge_sizes_and_iteration_counts():
         ref = oracle.mlem_batch_f32(C0[None], a, e[:, None], t, I, 1, 0.0, 0.01)[0]
         assert rel_l2(p.C.cpu().numpy(), ref) <= RTOL, shapes[i]
     run.close()
+
+
+def test_batch_spread_angles():
+    """Grouped launches with angles in random order over [0, 2 pi) on images
+    wider than two 64-column strips (footprint merges of far-apart angles)."""
+    shapes = [(12, 1, 200, 200), (9, 2, 150, 220)]
+    data = []
+    for i, (A, B, H, W) in enumerate(shapes):
+        atten, em, theta, I, C0 = _problem(A, B, H, W, seed=400 + i)
+        theta = np.random.default_rng(410 + i).uniform(0, 2 * np.pi, A).astype(np.float32)
+        C_true = np.random.default_rng(420 + i).uniform(0.0, 1.0, (1, B, H, W)).astype(np.float32)
+        I = oracle.forward_batch_f32(atten, C_true, em[:, None], theta)
+        data.append((atten, em, theta, I, C0))
+    print(f"batch spread angles {shapes}", flush=True)
+    run, probs = _batch(data)
+    run.run(3, graph=True)
+    torch.cuda.synchronize()
+    for i, (d, p) in enumerate(zip(data, probs)):
+        a, e, t, I, C0 = d
+        ref = oracle.mlem_batch_f32(C0[None], a, e[:, None], t, I, 3, 0.0, 0.01)[0]
+        assert rel_l2(p.C.cpu().numpy(), ref) <= RTOL, shapes[i]
+    run.close()

@@ -94,6 +94,27 @@ def test_adjoint_dot_product(ops):
     assert abs(lhs - rhs) / abs(lhs) < 1e-5
 
 
+@pytest.mark.parametrize("shape", [(16, 2, 1, 256, 256), (12, 2, 3, 200, 150)])
+def test_spread_angles_wide_images(ops, shape):
+    """Angles in random order over [0, 2 pi) on images wider than two 64-column
+    strips: the forward merges the footprints of a group's angles, and a merge
+    of far-apart angles must not outgrow the staged width.  Results must match
+    the oracle and repeat bit for bit."""
+    A, B, R, H, W = shape
+    th = np.random.default_rng(31).uniform(0, 2 * np.pi, A)
+    atten, C, em, theta, ratio = projector_inputs(A, B, R, H, W, seed=13, thetas=th)
+    print(f"spread angles {shape}", flush=True)
+    a, c, e, t, r = (_dev(x) for x in (atten, C, em, theta, ratio))
+    f1 = ops.forward_emission_batch(a, c, e, t)
+    f2 = ops.forward_emission_batch(a, c, e, t)
+    assert torch.equal(f1, f2)
+    assert rel_l2(f1.cpu().numpy(), oracle.forward_batch_f32(atten, C, em, theta)) <= RTOL
+    b1 = ops.backward_emission_batch(a, r, e, t)
+    b2 = ops.backward_emission_batch(a, r, e, t)
+    assert torch.equal(b1, b2)
+    assert rel_l2(b1.cpu().numpy(), oracle.backward_batch_f32(atten, ratio, em, theta)) <= RTOL
+
+
 @pytest.mark.parametrize("beta", [0.0, 1e-3])
 def test_mlem_batch(ops, beta):
     # reduced config-0 shape: 2 references would share a line; here one line, 20 iterations
