@@ -298,9 +298,10 @@ __device__ __forceinline__ void slope_interval(float slope, float lo, float hi, 
     const float L = lo - base, U = hi - base;
     const float as = fabsf(slope);
     if (as > 0.0f) {
-        // approximate quotients: their error is far inside the margin m
+        // approximate quotients: their error (below 1e-4 here) is far inside the
+        // margin m
         const float r = __frcp_rn(slope);
-        const float m = 1.0f + tol * fabsf(r);
+        const float m = 0.01f + tol * fabsf(r);
         const float a = (slope > 0.0f ? L : U) * r, z = (slope > 0.0f ? U : L) * r;
         qmin = fmaxf(qmin, a - m);
         qmax = fminf(qmax, z + m);
@@ -358,12 +359,11 @@ struct BwdPair {
     float ew[kBRefChunk];
 };
 
+// c, s: cosf and sinf of the angle's theta.
 template <int NR>
-__device__ __forceinline__ void bwd_begin_angle(const BwdArgs &p, const BwdTile &t, BwdCursor &cur)
+__device__ __forceinline__ void bwd_begin_angle(const BwdArgs &p, const BwdTile &t, BwdCursor &cur, float c, float s)
 {
     const float cx = (p.W - 1) * 0.5f, cy = (p.H - 1) * 0.5f;
-    const float tha = finite_or_zero(scalar_read(p.theta, cur.a));
-    const float c = cosf(tha), s = sinf(tha);
     // backward_emission_batch_kernel (.cu:416-417):
     //   col = c*(q-cx) - s*(h-cy) + cx,  row = s*(q-cx) + c*(h-cy) + cy
     // Preimage rows: invert the rotation at the region's corners.
@@ -840,8 +840,17 @@ template <int NR, bool RATIO>
 __device__ __forceinline__ void bwd_stream(const BwdArgs &p, const BwdTile &t, int wave)
 {
     BwdCursor cur;
+    float cs_c = 0.0f, cs_s = 0.0f;  // lane l: angle a0 + 4l of the current block of 64
     for (cur.a = wave; cur.a < p.A; cur.a += kBWaves) {
-        bwd_begin_angle<NR>(p, t, cur);
+        const int j = ((cur.a - wave) / kBWaves) & 63;  // wave-uniform
+        if (j == 0) {
+            const int aa = cur.a + kBWaves * t.lane;
+            const float th = aa < p.A ? finite_or_zero(p.theta[aa]) : 0.0f;
+            cs_c = cosf(th);
+            cs_s = sinf(th);
+        }
+        bwd_begin_angle<NR>(p, t, cur, __int_as_float(__builtin_amdgcn_readlane(__float_as_int(cs_c), j)),
+                            __int_as_float(__builtin_amdgcn_readlane(__float_as_int(cs_s), j)));
         if (t.edges & kEdgeTop) bwd_edge<NR, RATIO, kEdgeTop>(p, t, cur);
         if (t.edges & kEdgeBottom) bwd_edge<NR, RATIO, kEdgeBottom>(p, t, cur);
         if (t.edges & kEdgeLeft) bwd_edge<NR, RATIO, kEdgeLeft>(p, t, cur);
