@@ -731,7 +731,7 @@ __device__ __forceinline__ void bwd_edge(const BwdArgs &p, const BwdTile &t, con
                 e.fr[i] = __builtin_amdgcn_fractf(X);
             }
             in = in && nn + i <= nb;
-            e.av[i] = load_f32(cur.ra, in ? (uint32_t)(h * p.W + q) * 4u : kOOB);
+            e.av[i] = load_f32(cur.ra, in ? (__umul24((uint32_t)h, (uint32_t)p.W) + (uint32_t)q) * 4u : kOOB);
             e.rv[i] = RATIO ? load_f32(cur.rr, in ? (uint32_t)q * 4u : kOOB) : 1.0f;
         }
     };
@@ -758,30 +758,41 @@ __device__ __forceinline__ void bwd_edge(const BwdArgs &p, const BwdTile &t, con
             }
         }
     };
-    int n, nb;
-    if (m <= mend) nrange(m, n, nb);
-    // skip empty steps
-    const auto advance = [&]() {
-        while (m <= mend && n > nb) {
-            ++m;
-            if (m <= mend) nrange(m, n, nb);
+    // One step per iteration: its points n0 and n0+1, the next step's loads in
+    // flight (two pair buffers used in turn, so no register copy waits on a
+    // load).  A step holds more than 2 points only where a strip reaches into a
+    // corner region (top/bottom column 0 or W-2); those extra points are taken
+    // on the spot.
+    const auto extra = [&](int mm, int na, int nb) {
+        for (int nn = na + 2; nn <= nb; nn += 2) {
+            EdgePair x;
+            gather(mm, nn, nb, x);
+            add(x);
         }
     };
-    // two pair buffers used in turn, so no register copy waits on a load in flight
-    advance();
+    int na, nb;
     EdgePair e0, e1;
-    if (m <= mend) gather(m, n, nb, e0);
+    if (m <= mend) {
+        nrange(m, na, nb);
+        gather(m, na, nb, e0);
+    }
     while (m <= mend) {
-        n += 2;
-        advance();
+        extra(m, na, nb);
+        ++m;
         const bool more = m <= mend;
-        if (more) gather(m, n, nb, e1);
+        if (more) {
+            nrange(m, na, nb);
+            gather(m, na, nb, e1);
+        }
         add(e0);
         if (!more) break;
-        n += 2;
-        advance();
+        extra(m, na, nb);
+        ++m;
         const bool more2 = m <= mend;
-        if (more2) gather(m, n, nb, e0);
+        if (more2) {
+            nrange(m, na, nb);
+            gather(m, na, nb, e0);
+        }
         add(e1);
         if (!more2) break;
     }
