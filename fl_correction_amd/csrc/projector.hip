@@ -150,7 +150,7 @@ __device__ __forceinline__ void fwd_strip_tile(const FwdArgs &p, uint32_t t)
             while (gj < na) {
                 const int mrlo = min(rlo, lane_value(mine.rlo, gj)), mrhi = max(rhi, lane_value(mine.rhi, gj));
                 const int mclo = min(clo, lane_value(mine.clo, gj)), mchi = max(chi, lane_value(mine.chi, gj));
-                if ((mrhi - mrlo + 1) * ((mchi - mclo + 1) | 1) * NR > kFLds || mchi - mclo + 1 > kFStageCols) break;
+                if ((mrhi - mrlo + 1) * ((mchi - mclo + 1) | 1) > kFLdsRef || mchi - mclo + 1 > kFStageCols) break;
                 rlo = mrlo;
                 rhi = mrhi;
                 clo = mclo;
@@ -1071,12 +1071,12 @@ int launch_forward(const float *atten, const float *C, const float *em, const fl
                    float *out, int64_t A, int64_t B, int64_t R, int64_t H, int64_t W, int mode, hipStream_t stream)
 {
     static bool attr_set = false;
-    const size_t shmem = sizeof(float) * kFLds;
+    const size_t shmem1 = fwd_lds_bytes(1), shmem2 = fwd_lds_bytes(2);
     if (!attr_set) {
         FL_HIP_CHECK(hipFuncSetAttribute((const void *)fwd_strip_kernel<1, kFG>, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                         (int)shmem));
+                                         (int)shmem1));
         FL_HIP_CHECK(hipFuncSetAttribute((const void *)fwd_strip_kernel<2, kFG>, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                         (int)shmem));
+                                         (int)shmem2));
         attr_set = true;
     }
     FwdArgs p{};
@@ -1103,9 +1103,9 @@ int launch_forward(const float *atten, const float *C, const float *em, const fl
         p.mode = first ? (last && ratio ? kFwdRatio : kFwdWrite)
                        : (last && ratio ? kFwdAccumulateRatio : kFwdAccumulate);
         if (p.nref == 2)
-            fwd_strip_kernel<2, kFG><<<dim3((uint32_t)nwg), dim3(256), shmem, stream>>>(p);
+            fwd_strip_kernel<2, kFG><<<dim3((uint32_t)nwg), dim3(256), shmem2, stream>>>(p);
         else
-            fwd_strip_kernel<1, kFG><<<dim3((uint32_t)nwg), dim3(256), shmem, stream>>>(p);
+            fwd_strip_kernel<1, kFG><<<dim3((uint32_t)nwg), dim3(256), shmem1, stream>>>(p);
         FL_LAUNCH_CHECK();
     }
     return FL_OK;
@@ -1177,7 +1177,7 @@ static int resident_grid(const void *kernel, size_t lds, int *grid)
     return FL_OK;
 }
 
-static const size_t kFwdGroupLds = sizeof(float) * kFLds;
+static const size_t kFwdGroupLds = fwd_lds_bytes(1);
 static const size_t kBwdGroupLds = bwd_lds_bytes(1, 31);  // the larger of the two tile heights
 
 int fwd_group_grid(int *grid)

@@ -14,12 +14,18 @@ constexpr int kFH = 32;                   // detector rows per chunk
 constexpr int kFWaves = 4;
 constexpr int kFRows = kFH / kFWaves;     // rows per wave per chunk
 constexpr int kFG = 8;                    // angles per workgroup (one accumulator each)
-constexpr int kFLds = 11264;              // footprint floats staged per workgroup (44 KB)
 // One angle's footprint for a 64x32 tile: the sample points span at most
 // hypot(31, 63) < 70.3 rows (and cols); with the 0.5 margins and the bilinear
 // neighbour that is <= 74 rows and a pitch <= 75.
 constexpr int kFMaxFootprint = 74 * 75;
-constexpr int kFRefChunk = kFLds / kFMaxFootprint;  // references staged together (2)
+// Footprint floats staged per reference (24 KB): one angle always fits, and the
+// union of a group of nearby angles usually does; the small budget keeps four
+// workgroups per CU for one reference.
+constexpr int kFLdsRef = 6144;
+static_assert(kFMaxFootprint <= kFLdsRef, "one angle's footprint must fit");
+static_assert(kFG * 4 * 64 <= kFLdsRef, "the final reduction reuses the staging area");
+constexpr int kFRefChunk = 2;             // references staged together
+constexpr size_t fwd_lds_bytes(int nref) { return sizeof(float) * (size_t)nref * kFLdsRef; }
 // Footprint columns the staging covers (two per lane); a union of several
 // angles' footprints may not be wider.  One angle's is at most 75.
 constexpr int kFStageCols = 2 * 64;
