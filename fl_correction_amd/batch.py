@@ -176,11 +176,18 @@ class Autotuner:
     def shapes(self, A, B, H, W, device=None):
         device = torch.device(device or "cuda")
         k = self.key(torch.cuda.get_device_name(device), H, W)
-        if k not in self.table:
+        if not self.valid(self.table.get(k)):  # missing, or from another library version
             self.table[k] = list(self._tune(*self.bucket(H, W), device))
             self._save()
         fa, br = self.table[k]
         return int(fa), int(br)
+
+    @staticmethod
+    def valid(entry):
+        """A cached choice this library accepts: [angles per forward workgroup,
+        rows per back-projector tile]."""
+        return (isinstance(entry, (list, tuple)) and len(entry) == 2 and entry[0] in FWD_ANGLES
+                and entry[1] in BWD_ROWS)
 
     def _save(self):
         try:

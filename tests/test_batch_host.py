@@ -73,5 +73,8 @@ def test_autotuner_buckets_and_cache(tmp_path):
     path.write_text(json.dumps({"X|64x64": [4, 15]}))
     tu = Autotuner(path=str(path))
     assert tu.table["X|64x64"] == [4, 15]
+    # choices the library does not accept are tuned again, not used
+    assert Autotuner.valid([4, 15]) and Autotuner.valid([8, 31])
+    assert not Autotuner.valid([4, 16]) and not Autotuner.valid([5, 31]) and not Autotuner.valid(None)
     (tmp_path / "bad.json").write_text("{not json")
     assert Autotuner(path=str(tmp_path / "bad.json")).table == {}
