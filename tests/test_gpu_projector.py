@@ -94,12 +94,14 @@ def test_adjoint_dot_product(ops):
     assert abs(lhs - rhs) / abs(lhs) < 1e-5
 
 
-@pytest.mark.parametrize("shape", [(16, 2, 1, 256, 256), (12, 2, 3, 200, 150)])
+@pytest.mark.parametrize("shape", [(16, 2, 1, 256, 256), (12, 2, 3, 200, 150), (20, 2, 1, 24, 160), (20, 2, 2, 160, 24)])
 def test_spread_angles_wide_images(ops, shape):
     """Angles in random order over [0, 2 pi) on images wider than two 64-column
     strips: the forward merges the footprints of a group's angles, and a merge
-    of far-apart angles must not outgrow the staged width.  Results must match
-    the oracle and repeat bit for bit."""
+    of far-apart angles must not outgrow the staged width.  The elongated
+    shapes put many clamped samples into the image corners (the back-projector's
+    border gather walks strips that reach into the corner regions).  Results
+    must match the oracle and repeat bit for bit."""
     A, B, R, H, W = shape
     th = np.random.default_rng(31).uniform(0, 2 * np.pi, A)
     atten, C, em, theta, ratio = projector_inputs(A, B, R, H, W, seed=13, thetas=th)
