@@ -289,7 +289,7 @@ __device__ __forceinline__ float huber_at(const float *Cs, int i, int j, int H, 
 }
 
 // Range of qc with lo <= fmaf(slope, qc, base) < hi, intersected into [qmin, qmax].
-// The real-arithmetic bounds are widened by one plus the distance the rounding
+// The real-arithmetic bounds are widened by 0.01 plus the distance the rounding
 // of the fma (at most tol in value) can move them, so the range always contains
 // every qc that passes the exact test; the caller tightens it point by point.
 __device__ __forceinline__ void slope_interval(float slope, float lo, float hi, float base, float tol,
@@ -628,7 +628,8 @@ __device__ __forceinline__ void bwd_edge(const BwdArgs &p, const BwdTile &t, con
     const bool active = t.lane < nlanes && k >= 0 && k <= kmax;
     // Walk the detector axis along the strip (major m), the other one per step
     // (minor n): the strip coordinate P (Y for top/bottom, X for left/right)
-    // changes by at least 0.7 per minor step, so a step holds at most 2 points.
+    // changes by at least 0.7 per minor step, so a step of a strip one cell wide
+    // holds at most 2 points (the corner cells' strips are open on one side).
     //   X - cy = c*hc + s*qc,  Y - cx = -s*hc + c*qc   (hc = h - cy, qc = q - cx)
     const bool major_h = xedge ? fabsf(c) >= fabsf(s) : fabsf(c) < fabsf(s);
     const float cm = major_h ? cy : cx, cn = major_h ? cx : cy;
