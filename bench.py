@@ -391,7 +391,7 @@ def cpu_attenuation_sample(args):
     ncores = len(os.sched_getaffinity(0))
     os.environ.setdefault("OMP_NUM_THREADS", str(ncores))
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
-    import oracle  # the checker's CPU restatement, used here as the timed CPU port
+    import oracle  # the oracle's CPU restatement, used here as the timed CPU port
     from fl_correction_amd.detector_mask import Cone
 
     N, K, Zs = args.size, args.elements, 4
@@ -417,7 +417,7 @@ def cpu_baseline(args):
     ncores = len(os.sched_getaffinity(0))
     os.environ.setdefault("OMP_NUM_THREADS", str(ncores))
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
-    import oracle  # the checker's CPU restatement, used here as the timed CPU port
+    import oracle  # the oracle's CPU restatement, used here as the timed CPU port
 
     N = args.size
     ka = min(args.cpu_angles, args.angles)

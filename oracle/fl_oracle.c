@@ -3,7 +3,7 @@
  *
  * TEST INFRASTRUCTURE ONLY.  Nothing in the product (fl_correction_amd/) links,
  * loads or calls this file; only tests/, __graft_entry__.smoke() and the
- * cpu_baseline leg of bench.py do, and only as the checker.
+ * cpu_baseline leg of bench.py do, and only to check results.
  *
  * Two families of functions:
  *   *_f32  "same-order" FP32 restatements: every loop runs in the order the

@@ -1,7 +1,7 @@
 """CPU oracle for the XRF self-absorption hot path -- TEST INFRASTRUCTURE ONLY.
 
 Only tests/, ``__graft_entry__.smoke()`` and the ``cpu_baseline`` leg of
-``bench.py`` import this module, and only as the checker (or, for the
+``bench.py`` import this module, and only to check results (or, for the
 baseline, as the timed CPU port).  The product package never imports it.
 
 Contents

@@ -6,7 +6,7 @@ import numpy as np
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 sys.path.insert(0, os.path.join(ROOT, "oracle"))
-import oracle  # noqa: E402  (oracle/oracle.py: the CPU checker)
+import oracle  # noqa: E402  (oracle/oracle.py: the CPU oracle)
 
 RTOL = 1e-5  # relative L2 bound of every FP32 result against the same-order FP32 oracle
 
