@@ -276,8 +276,19 @@ def model_timings(args, C_true, pix):
     G = torch.randn_like(Pf)
     t_vjp, _ = timed(lambda: model.vjp(Cm, G))
     model.release()
+    del model
+    # The same with the fluorescence exponents kept from forward to vjp.
+    smodel = XRFModel(param, angles, cone, store=True)
+    t_sfwd, _ = timed(lambda: smodel.forward(Cm))
+    t_svjp, _ = timed(lambda: smodel.vjp(Cm, G))
+    smodel.release()
+    del smodel
     per = lambda t: t / na
-    log(f"model: {na} angles: attenuation {t_att:.1f} ms, forward {t_fwd:.1f} ms, vjp {t_vjp:.1f} ms")
+    log(f"model: {na} angles: attenuation {t_att:.1f} ms, forward {t_fwd:.1f} ms, vjp {t_vjp:.1f} ms; "
+        f"exponents kept: forward {t_sfwd:.1f} ms, vjp {t_svjp:.1f} ms")
+    B, H, W = Cm.shape[1:]
+    store_bytes = 4.0 * A * K * B * H * W  # all angles' exponents
+    total = torch.cuda.get_device_properties(0).total_memory
     return {
         "angles_timed": na, "cone": "5x5 directions, 15 deg half-angle, slopes in 1/8",
         "attenuation_ms_per_angle": per(t_att),
@@ -288,6 +299,16 @@ def model_timings(args, C_true, pix):
         "grad_evals_per_s_extrapolated": 1000.0 / (per(t_fwd + t_vjp) * A),
         "note": "forward recomputes the attenuation for every angle (nothing stored); vjp recomputes it too "
                 "and differentiates through it",
+        "exponents_kept": {
+            "forward_ms_per_angle": per(t_sfwd),
+            "vjp_ms_per_angle": per(t_svjp),
+            "adjoint_over_forward": t_svjp / t_sfwd,
+            "grad_evals_per_s_extrapolated": 1000.0 / (per(t_sfwd + t_svjp) * A),
+            "stored_gb_at_all_angles": store_bytes / 1e9,
+            "fits_default_budget": bool(store_bytes <= 0.4 * total),
+            "note": "forward keeps every angle's fluorescence exponents and vjp reuses them (the model's "
+                    "store mode; its default 'auto' keeps them when they fit 40% of free HBM)",
+        },
     }
 
 
