@@ -57,12 +57,19 @@ def mlem_start(model: XRFModel, I, C_init, n_iter=10, beta=0.0, delta=0.01):
 
 
 def self_consistent_mlem(model: XRFModel, I, C_init, n_iter=50, recompute_every=5, beta=0.0, delta=0.01,
-                         callback=None):
+                         callback=None, start_unattenuated=False):
     """n_iter MLEM iterations per emission line, the attenuation recomputed from
     the current estimate before every block of `recompute_every` iterations.
 
     I (K, A, B, W), C_init (K, B, H, W); returns a new (K, B, H, W) tensor.
-    callback(iteration, C) is called after every block if given."""
+    callback(iteration, C) is called after every block if given.
+
+    start_unattenuated: the first block runs without attenuation (atten = 1),
+    as the reference's demo makes its first reconstruction (FL/example/demo.py:
+    186-228, atten=None) before it recomputes attenuation from an estimate.  A
+    uniform start over the whole volume attenuates far more than a sparse
+    sample does, and MLEM then raises the shadowed voxels until the next
+    recompute overflows; the unattenuated first block avoids that."""
     if recompute_every < 1:
         raise ValueError("recompute_every must be >= 1")
     C = C_init.to(model.device, torch.float32).clone().contiguous()
@@ -70,7 +77,12 @@ def self_consistent_mlem(model: XRFModel, I, C_init, n_iter=50, recompute_every=
     done = 0
     while done < n_iter:
         k = min(int(recompute_every), int(n_iter) - done)
-        atten = model.attenuation(C)  # (K, A, B, H, W) for the current estimate
+        if start_unattenuated and done == 0:
+            # atten=None: ones, one (A, B, H, W) array shared by every line
+            ones = torch.ones((model.A, *C.shape[1:]), dtype=torch.float32, device=model.device)
+            atten = [ones] * model.K
+        else:
+            atten = model.attenuation(C)  # (K, A, B, H, W) for the current estimate
         for e in range(model.K):
             em = model.em_T[e][:, None].contiguous()
             solver = MLEMSolver(atten[e], em, model.theta, I[e].contiguous(), beta=beta, delta=delta)

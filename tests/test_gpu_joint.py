@@ -73,3 +73,38 @@ def test_self_consistent_mlem_matches_oracle_composition():
     err = rel_l2(got, C)
     print(f"  vs oracle composition: rel-L2 {err:.2e}")
     assert err <= RTOL
+
+
+def test_self_consistent_mlem_unattenuated_start():
+    """start_unattenuated: the first block is plain MLEM with atten = 1 (the
+    reference demo's atten=None reconstruction); the second block recomputes the
+    attenuation from that estimate (checked against the loop without the option
+    started from the first block's result)."""
+    from helpers import RTOL, oracle, rel_l2
+    from fl_correction_amd import phantom
+    from fl_correction_amd.atten_util import make_param
+    from fl_correction_amd.detector_mask import generate_detector_mask
+    from fl_correction_amd.joint import self_consistent_mlem
+    from fl_correction_amd.model import XRFModel
+    K, B, N, A = 2, 2, 16, 6
+    rng = np.random.default_rng(20260011)
+    C_true = phantom.ellipsoid_phantom((B, N, N), K, n_ell=(2, 4), semi_axes=(2, 6), seed=20260011)
+    param = make_param(["Fe", "Cu"], ["Fe", "Cu"], rng.uniform(20, 400, K), rng.uniform(20, 400, (K, K)),
+                       rho=1.0, pix=1e-4)
+    mask = generate_detector_mask(20, 20, N).astype(np.float32)
+    angles = phantom.angle_list(A)
+    model = XRFModel(param, angles, mask, store=False)
+    I = model.forward(torch.from_numpy(C_true).cuda())
+    C0 = torch.ones((K, B, N, N), device="cuda")
+    print(f"self-consistent MLEM, unattenuated start: K={K} B={B} N={N} A={A}", flush=True)
+    first = self_consistent_mlem(model, I, C0, n_iter=3, recompute_every=3, start_unattenuated=True)
+    I_np, em, theta = I.cpu().numpy(), model.em_T.cpu().numpy(), model.theta.cpu().numpy()
+    ones = np.ones((A, B, N, N), np.float32)
+    for e in range(K):
+        ref = oracle.mlem_batch_f32(np.ones((1, B, N, N), np.float32), ones, em[e][:, None].copy(), theta, I_np[e],
+                                    3, 0.0, 0.01)[0]
+        assert rel_l2(first[e].cpu().numpy(), ref) <= RTOL
+    both = self_consistent_mlem(model, I, C0, n_iter=6, recompute_every=3, start_unattenuated=True)
+    then = self_consistent_mlem(model, I, first, n_iter=3, recompute_every=3)
+    assert rel_l2(both.cpu().numpy(), then.cpu().numpy()) <= 1e-6
+
