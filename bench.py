@@ -208,6 +208,7 @@ def main():
 
     if not args.no_cpu_baseline:
         result["cpu_baseline"] = cpu_baseline(args)
+        result["cpu_baseline"]["legacy_dense"] = legacy_dense_small(args)
         if "model" in result:
             # the iteration that recomputes the attenuation for every angle first
             m = result["model"]
@@ -465,6 +466,74 @@ def cpu_baseline(args):
     return {"value": 1.0 / (dt * scale), "unit": "iter/s", "cores": ncores, "kind": "port",
             "sample": f"one line, {ka} of {args.angles} angles at {N}^3: median of 3 runs {dt:.2f}s on {ncores} threads, "
                       f"scaled x{scale:g} (lines are independent)"}
+
+
+def legacy_dense_small(args, N=64, A=60, K=2, n_slices=8):
+    """The reference's legacy CPU path (dense system matrix per slice, FP64
+    generate_H_jit + mlem_matrix, restated in the oracle) on its own CPU-sized
+    case, 64^3 x 60 angles x 2 lines, beside this library's MLEM iteration on
+    the GPU at the same size.  CPU: n_slices slices of one line, median of 3
+    runs, scaled to all slices and lines."""
+    import torch
+    from fl_correction_amd.mlem import MLEMSolver
+    ncores = len(os.sched_getaffinity(0))
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle  # the oracle's legacy restatement, timed as the legacy-faithful CPU path
+
+    rng = np.random.default_rng(20260000)
+    theta = np.deg2rad(np.linspace(0, 180, A, endpoint=False))
+    slices = [rng.uniform(0.3, 1.0, (A, N, N)) for _ in range(n_slices)]
+    y = [rng.uniform(0.5, 2.0, A * N) for _ in range(n_slices)]
+    c0 = np.ones((N, N), np.float32)
+    t_h, t_it = [], []
+    for _ in range(3):
+        th = ti = 0.0
+        for k in range(n_slices):
+            t0 = time.perf_counter()
+            Hm = oracle.legacy_H_f64(slices[k], theta)
+            t1 = time.perf_counter()
+            oracle.legacy_mlem_f64(Hm, y[k], c0, 1)
+            t2 = time.perf_counter()
+            oracle.legacy_mlem_f64(Hm, y[k], c0, 3)
+            t3 = time.perf_counter()
+            # per iteration: two more iterations; the column sums b_sum cancel
+            th, ti = th + (t1 - t0), ti + ((t3 - t2) - (t2 - t1)) / 2
+            del Hm
+        t_h.append(th)
+        t_it.append(ti)
+    scale = (N / n_slices) * K
+    cpu_h, cpu_it = float(np.median(t_h)) * scale, float(np.median(t_it)) * scale
+
+    dev = torch.device("cuda:0")
+    th_t = torch.tensor(theta, dtype=torch.float32, device=dev)
+    em = torch.ones((A, 1), dtype=torch.float32, device=dev)
+    sol = [MLEMSolver(torch.rand((A, N, N, N), device=dev) * 0.7 + 0.3, em, th_t,
+                      torch.rand((A, N, N), device=dev) + 0.5) for _ in range(K)]
+    C = [torch.ones((1, N, N, N), device=dev) for _ in range(K)]
+    Cn = [torch.empty_like(c) for c in C]
+
+    def it():
+        for e in range(K):
+            sol[e].ratio_step(C[e])
+            sol[e].update_step(C[e], Cn[e])
+            C[e], Cn[e] = Cn[e], C[e]
+
+    it()
+    torch.cuda.synchronize()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for _ in range(20):
+        it()
+    e1.record()
+    torch.cuda.synchronize()
+    gpu_ms = e0.elapsed_time(e1) / 20
+    log(f"legacy dense CPU path at {N}^3 x {A} x {K}: H {cpu_h:.2f}s, {cpu_it:.2f}s per iteration; "
+        f"GPU {gpu_ms:.3f} ms per iteration")
+    return {"workload": f"{N}^3 voxels x {A} angles x {K} lines (the reference's CPU-sized case)",
+            "cpu_h_build_s": cpu_h, "cpu_s_per_iteration": cpu_it, "gpu_ms_per_iteration": gpu_ms,
+            "speedup_per_iteration": cpu_it / (gpu_ms * 1e-3), "cores": ncores,
+            "sample": f"{n_slices} of {N} slices of one line, median of 3 runs, scaled x{scale:g}; FP64 dense "
+                      f"matrix of {A * N} x {N * N} per slice; per iteration = (3 iterations - 1 iteration) / 2"}
 
 
 if __name__ == "__main__":

@@ -588,3 +588,93 @@ void or_cone_march_f64(const double *mu, double *out, int Z, int X, int Y, const
                 out[((int64_t)z * X + x) * Y + y] = acc;
             }
 }
+
+/* ------------------------------------------------------------------------- */
+/* Legacy CPU path, FL/legacy/FL_correction_core.py, FP64 (timed as the      */
+/* legacy-faithful CPU baseline, and a second statement of the projector).   */
+/*                                                                           */
+/* or_legacy_H_f64: generate_H_jit :2203-2279, the dense system matrix of    */
+/* one slice: row i*W + q (angle i, detector column q), column r*W + c.      */
+/* Detector point (p = row, q = column) samples the image at                 */
+/*   t0 = cos(-th)(p - cx) - sin(-th)(q - cy) + cx   (row)                   */
+/*   t1 = sin(-th)(p - cx) + cos(-th)(q - cy) + cy   (column)                */
+/* with cx = (W-1)/2, cy = (H-1)/2 paired as the reference pairs them        */
+/* (:2234-2235, :2252-2253; the same as the projector's for square slices).  */
+/* Points outside [0, H-1] x [0, W-1] are skipped (:2255-2256; the GPU       */
+/* projector clamps them instead); inside ones add att[p,q] times the        */
+/* bilinear weights of the floor/ceil corners, in the order ul, dl, ur, dr   */
+/* (:2258-2276).  Loops: angles, then columns, then rows (:2239, :2247-2248).*/
+void or_legacy_H_f64(const double *theta, const double *atten, double *Hm, int A, int H, int W)
+{
+    const int64_t npix = (int64_t)H * W;
+    const double cx = (W - 1) / 2.0, cy = (H - 1) / 2.0;
+#pragma omp parallel for schedule(static)
+    for (int i = 0; i < A; ++i) {
+        const double T00 = cos(-theta[i]), T01 = -sin(-theta[i]);
+        const double T10 = sin(-theta[i]), T11 = cos(-theta[i]);
+        const double *att = atten + (int64_t)i * npix;
+        double *Hi = Hm + (int64_t)i * W * npix;
+        memset(Hi, 0, sizeof(double) * (size_t)(W * npix));
+        for (int q = 0; q < W; ++q)
+            for (int p = 0; p < H; ++p) {
+                const double t0 = T00 * (p - cx) + T01 * (q - cy) + cx;
+                const double t1 = T10 * (p - cx) + T11 * (q - cy) + cy;
+                if (t0 > H - 1 || t0 < 0 || t1 > W - 1 || t1 < 0) continue;
+                const double rf = t0 - floor(t0), cf = t1 - floor(t1);
+                const int64_t ru = (int64_t)floor(t0), rd = (int64_t)ceil(t0);
+                const int64_t cl = (int64_t)floor(t1), cr = (int64_t)ceil(t1);
+                const double a = att[(int64_t)p * W + q];
+                double *row = Hi + (int64_t)q * npix;
+                row[ru * W + cl] += a * (1 - rf) * (1 - cf);
+                row[rd * W + cl] += a * rf * (1 - cf);
+                row[ru * W + cr] += a * (1 - rf) * cf;
+                row[rd * W + cr] += a * rf * cf;
+            }
+    }
+}
+
+/* or_legacy_mlem_f64: mlem_matrix :99-119 on one slice.  A = C0 (float32,   */
+/* :104), y clipped at 0 (:107), b_sum = 1^T p (:106, row_sum :248-252); per  */
+/* iteration Pf = p A, Pf < 1e-6 -> 1 (:111-112, not a clamp to 1e-6),       */
+/* a_sum = (y/Pf)^T p, A = A * a_sum / b_sum, NaN -> 0 (:113-117): pixels     */
+/* that no row samples (b_sum = 0) become 0.  No Huber term.  Products are   */
+/* summed in index order (the reference's BLAS order is not fixed).          */
+void or_legacy_mlem_f64(const double *Hm, const double *y_in, const float *C0, double *out, int64_t nrow,
+                        int64_t npix, int n_iter)
+{
+    double *bsum = calloc((size_t)npix, sizeof(double));
+    double *asum = malloc(sizeof(double) * (size_t)npix);
+    double *t2 = malloc(sizeof(double) * (size_t)nrow);
+    for (int64_t c = 0; c < npix; ++c) out[c] = (double)C0[c];
+    const int64_t cb = 256;  /* column blocks: each thread owns whole columns */
+#pragma omp parallel for schedule(static)
+    for (int64_t c0 = 0; c0 < npix; c0 += cb) {
+        const int64_t c1 = c0 + cb < npix ? c0 + cb : npix;
+        for (int64_t r = 0; r < nrow; ++r)
+            for (int64_t c = c0; c < c1; ++c) bsum[c] += Hm[r * npix + c];
+    }
+    for (int it = 0; it < n_iter; ++it) {
+#pragma omp parallel for schedule(static)
+        for (int64_t r = 0; r < nrow; ++r) {
+            double pf = 0.0;
+            for (int64_t c = 0; c < npix; ++c) pf += Hm[r * npix + c] * out[c];
+            if (pf < 1e-6) pf = 1.0;
+            const double y = y_in[r] < 0 ? 0.0 : y_in[r];
+            t2[r] = y / pf;
+        }
+#pragma omp parallel for schedule(static)
+        for (int64_t c0 = 0; c0 < npix; c0 += cb) {
+            const int64_t c1 = c0 + cb < npix ? c0 + cb : npix;
+            for (int64_t c = c0; c < c1; ++c) asum[c] = 0.0;
+            for (int64_t r = 0; r < nrow; ++r)
+                for (int64_t c = c0; c < c1; ++c) asum[c] += t2[r] * Hm[r * npix + c];
+            for (int64_t c = c0; c < c1; ++c) {
+                const double v = out[c] * (asum[c] / bsum[c]);
+                out[c] = isnan(v) ? 0.0 : v;
+            }
+        }
+    }
+    free(bsum);
+    free(asum);
+    free(t2);
+}

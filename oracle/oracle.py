@@ -415,6 +415,37 @@ def cone_march_f64(mu, pz, py, q, wgt):
     return out
 
 
+# ----------------------------------------------------------------------------
+# Legacy CPU path  (FL/legacy/FL_correction_core.py: generate_H_jit :2203-2279,
+# mlem_matrix :99-119), FP64 -- the legacy-faithful CPU baseline
+# ----------------------------------------------------------------------------
+def legacy_H_f64(atten, theta):
+    """Dense system matrix of one slice, (A*W, H*W): row i*W + q is detector
+    column q at angle i, samples outside the image skipped (not clamped).
+    atten (A, H, W), theta (A) radians."""
+    atten = _f64(atten)
+    A, H, W = atten.shape
+    th = _f64(theta)
+    Hm = np.empty((A * W, H * W), np.float64)
+    lib().or_legacy_H_f64(_p(th), _p(atten), _p(Hm), _i(A), _i(H), _i(W))
+    return Hm
+
+
+def legacy_mlem_f64(Hm, y, C0, n_iter):
+    """mlem_matrix on one slice: Hm (A*W, H*W), y (A*W) projections, C0 (H, W)
+    start (taken as float32); returns the (H, W) FP64 estimate."""
+    Hm = _f64(Hm)
+    nrow, npix = Hm.shape
+    y = _f64(np.ravel(y))
+    c0 = _f32(np.ravel(C0))
+    if y.size != nrow or c0.size != npix:
+        raise ValueError("y must have one value per row of Hm and C0 one per column")
+    out = np.empty(npix, np.float64)
+    lib().or_legacy_mlem_f64(_p(Hm), _p(y), _p(c0), _p(out), ctypes.c_int64(nrow), ctypes.c_int64(npix),
+                             _i(int(n_iter)))
+    return out.reshape(np.shape(C0))
+
+
 def rel_l2(x, y):
     x = np.asarray(x, np.float64)
     y = np.asarray(y, np.float64)

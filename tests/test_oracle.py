@@ -301,3 +301,72 @@ def test_detector_mask_invariants():
         nz = np.nonzero(m[:, d, zc])[0]
         if len(nz):
             assert np.abs(nz - zc).max() <= d * t + 1
+
+
+# ---------------------------------------------------------------------------
+# Legacy dense-matrix path (FL/legacy/FL_correction_core.py generate_H_jit,
+# mlem_matrix): the legacy-faithful CPU baseline
+# ---------------------------------------------------------------------------
+def _disc_slice(N, margin, seed):
+    rng = np.random.default_rng(seed)
+    r, c = np.mgrid[0:N, 0:N]
+    ctr = (N - 1) / 2
+    inside = (r - ctr) ** 2 + (c - ctr) ** 2 <= (ctr - margin) ** 2
+    return np.where(inside, rng.uniform(0.2, 1.0, (N, N)), 0.0)
+
+
+def test_legacy_H_theta_zero_is_column_sum():
+    """At theta = 0 every detector point samples its own pixel with weight 1."""
+    rng = np.random.default_rng(41)
+    N = 12
+    atten = rng.uniform(0.3, 1.0, (1, N, N))
+    C = rng.uniform(0.0, 1.0, (N, N))
+    Hm = oracle.legacy_H_f64(atten, np.zeros(1))
+    np.testing.assert_allclose(Hm @ C.ravel(), (atten[0] * C).sum(axis=0), rtol=1e-13, atol=0)
+
+
+def test_legacy_H_matches_projector_away_from_the_border():
+    """Where the slice is zero near its border, skipping outside points (legacy)
+    and clamping them (the projector) give the same projections."""
+    rng = np.random.default_rng(42)
+    N, A = 20, 9
+    theta = rng.uniform(0, np.pi, A).astype(np.float32)
+    atten = rng.uniform(0.3, 1.0, (A, N, N))
+    C = _disc_slice(N, 2.0, 43)
+    Hm = oracle.legacy_H_f64(atten, theta.astype(np.float64))
+    got = (Hm @ C.ravel()).reshape(A, N)
+    ref = oracle.forward_batch_f64(atten[:, None], C[None, None], np.ones((A, 1)), theta)[:, 0]
+    assert rel_l2(got, ref) <= 1e-12
+
+
+def test_legacy_mlem_fixed_point_and_unsampled_pixels():
+    """y = H x*: x* is a fixed point on sampled pixels; pixels no row samples
+    (b_sum = 0) become 0 (the reference's NaN -> 0)."""
+    rng = np.random.default_rng(44)
+    N, A = 16, 7
+    theta = rng.uniform(0.2, 1.3, A)  # no angle near 0: the corners go unsampled
+    atten = rng.uniform(0.3, 1.0, (A, N, N))
+    Hm = oracle.legacy_H_f64(atten, theta)
+    x = rng.uniform(0.5, 1.0, (N, N)).astype(np.float32)
+    y = Hm @ x.astype(np.float64).ravel()
+    got = oracle.legacy_mlem_f64(Hm, y, x, 3)
+    sampled = (Hm.sum(axis=0) > 0).reshape(N, N)
+    assert (~sampled).any() and sampled.any()
+    np.testing.assert_allclose(got[sampled], x[sampled], rtol=1e-10)
+    assert np.all(got[~sampled] == 0)
+
+
+def test_legacy_mlem_raises_the_poisson_likelihood():
+    rng = np.random.default_rng(45)
+    N, A = 16, 12
+    theta = np.linspace(0, np.pi, A, endpoint=False)
+    atten = rng.uniform(0.3, 1.0, (A, N, N))
+    Hm = oracle.legacy_H_f64(atten, theta)
+    y = rng.poisson(Hm @ _disc_slice(N, 1.0, 46).ravel() * 50) / 50.0
+    x = np.ones((N, N), np.float32)
+    ll = []
+    for n in range(1, 6):
+        est = oracle.legacy_mlem_f64(Hm, y, x, n).ravel()
+        pf = np.maximum(Hm @ est, 1e-12)
+        ll.append(float(np.sum(y * np.log(pf) - pf)))
+    assert all(b >= a - 1e-9 * abs(a) for a, b in zip(ll, ll[1:]))
