@@ -253,19 +253,29 @@ __global__ __launch_bounds__(256) void cone_kernel(const float *__restrict__ mu,
                                      (p.dz[threadIdx.x] & 0xffff) | (p.t[threadIdx.x] << 16));
     __syncthreads();
     const int lane = threadIdx.x & 63;
-    const int zi = blockIdx.y * 4 + uniform_int(threadIdx.x >> 6);
+    // Block order: the chain plane x0 fastest, then the y block, the z group and
+    // the line.  The q chains of one (y, z) tile read the same planes of mu, and
+    // neighbouring z groups share rows: remapped so that each XCD receives a
+    // contiguous run of this order, they run together and share its L2.
+    const uint32_t nyb = (uint32_t)(p.ny + 63) / 64u, nzb = (uint32_t)(p.nz + 3) / 4u;
+    uint32_t id = xcd_remap(blockIdx.x, gridDim.x);
+    const int x0 = (int)(id % (uint32_t)p.q);
+    id /= (uint32_t)p.q;
+    const int yb = (int)(id % nyb);
+    id /= nyb;
+    const int zb = (int)(id % nzb);
+    const int k = (int)(id / nzb);
+    const int zi = zb * 4 + uniform_int(threadIdx.x >> 6);
     if (zi >= p.nz) return;  // the whole wave
-    const int yi = blockIdx.x * 64 + lane;
+    const int yi = yb * 64 + lane;
     const bool act = yi < p.ny;
-    const int x0 = blockIdx.z % p.q;
-    const int k = blockIdx.z / p.q;
     const int64_t ZXY = (int64_t)Z * X * Y;
     const float *muk = mu + k * ZXY;
     const __amdgpu_buffer_rsrc_t rs = make_rsrc(muk, (int)ZXY);
     float *Tk = T + k * ZXY;
     const __amdgpu_buffer_rsrc_t rt = make_rsrc(Tk, (int)ZXY);
     const int z0 = p.zlo + zi;
-    const int yw0 = p.ylo + (int)blockIdx.x * 64;  // lane 0's chain start
+    const int yw0 = p.ylo + yb * 64;  // lane 0's chain start
     const int M = (X - x0 + p.q - 1) / p.q;        // chain points with x < X
     float J = 0.0f;
     for (int it = 0; it < M; ++it) {
@@ -477,8 +487,9 @@ int launch_cone_direction(const float *mu, float *T, int64_t K, int64_t Z, int64
     p.nz = (int)(zhi - zlo + 1);
     p.ylo = (int)ylo;
     p.ny = (int)(yhi - ylo + 1);
-    dim3 grid((unsigned)((p.ny + 63) / 64), (unsigned)((p.nz + 3) / 4), (unsigned)(K * q));
-    FL_REQUIRE(K * q <= 65535 && (p.nz + 3) / 4 <= 65535, "cone direction: grid too large");
+    const int64_t nblocks = (int64_t)((p.ny + 63) / 64) * ((p.nz + 3) / 4) * K * q;
+    FL_REQUIRE(nblocks < ((int64_t)1 << 31), "cone direction: grid too large");
+    dim3 grid((unsigned)nblocks);
     switch (NT) {
     case 4: cone_kernel<4><<<grid, dim3(256), 0, s>>>(mu, T, (int)Z, (int)X, (int)Y, p); break;
     case 8: cone_kernel<8><<<grid, dim3(256), 0, s>>>(mu, T, (int)Z, (int)X, (int)Y, p); break;
