@@ -276,7 +276,13 @@ int launch_rotate_adjoint(const float *dF, float *dC, int64_t N, int64_t H, int6
     const float rj = ac + as * (float)W / (float)H + 0.01f;
     const float ri = ac + as * (float)H / (float)W + 0.01f;
     FL_REQUIRE((2 * rj + 1) * (2 * ri + 1) <= 25.0f, "rotate_adjoint: aspect ratio too large");
-    dim3 grid((unsigned)((W + 63) / 64), (unsigned)((H + 3) / 4), (unsigned)std::min<int64_t>(N, 65535));
+    // The candidate weights depend only on the pixel: each block finds them once
+    // and applies them to N / grid.z planes.  grid.z is as small as keeps about
+    // kBlocks blocks in flight (enough waves to hide the loads' latency).
+    constexpr int64_t kBlocks = 16384;
+    const int64_t nxy = (int64_t)((W + 63) / 64) * ((H + 3) / 4);
+    const int64_t gz = std::max<int64_t>(1, std::min<int64_t>(N, (kBlocks + nxy - 1) / nxy));
+    dim3 grid((unsigned)((W + 63) / 64), (unsigned)((H + 3) / 4), (unsigned)std::min<int64_t>(gz, 65535));
     hipLaunchKernelGGL(rotate_adjoint_kernel, grid, dim3(256), 0, s, dF, dC, N, (int)H, (int)W, c, s_, rj, ri,
                        accumulate);
     FL_LAUNCH_CHECK();
