@@ -646,6 +646,13 @@ void or_legacy_mlem_f64(const double *Hm, const double *y_in, const float *C0, d
     double *asum = malloc(sizeof(double) * (size_t)npix);
     double *t2 = malloc(sizeof(double) * (size_t)nrow);
     for (int64_t c = 0; c < npix; ++c) out[c] = (double)C0[c];
+    if (!bsum || !asum || !t2) {  /* out of memory: NaN marks the result invalid */
+        for (int64_t c = 0; c < npix; ++c) out[c] = NAN;
+        free(bsum);
+        free(asum);
+        free(t2);
+        return;
+    }
     const int64_t cb = 256;  /* column blocks: each thread owns whole columns */
 #pragma omp parallel for schedule(static)
     for (int64_t c0 = 0; c0 < npix; c0 += cb) {
