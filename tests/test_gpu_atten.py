@@ -119,6 +119,7 @@ def test_cone_equals_splat_mask_and_march():
     (5, 15.0, 8, (2, 20, 40, 150)),    # q = 8, 4, 1 stencils; y wider than one wave (unchecked path)
     (3, 40.0, 4, (1, 13, 33, 200)),    # slopes near 1: chains leave through the z and y faces
     (3, 20.0, 16, (1, 9, 50, 130)),    # 64-tap stencils
+    (5, 15.0, 8, (3, 37, 24, 131)),    # three lines, odd sizes: partial y blocks and z groups
 ])
 def test_cone_wide_volumes(n, half, q, shape):
     from fl_correction_amd import _lib
