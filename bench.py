@@ -5,8 +5,15 @@ Workload: 256^3 voxels x 360 angles x 4 emission lines (one reference per line),
 FP32, attenuation resident in HBM (the form mlem_cuda_batch iterates on,
 FL/cuda_lib/atten_cuda.cpp:283-345).  One step = one plain MLEM iteration of
 all four lines: forward projection + ratio, then back-projection + update, per
-line.  Inputs are synthetic: seeded ellipsoid phantoms, attenuation maps and
-Poisson projections at 1e4 peak counts.
+line.  The lines are independent problems, so each runs on its own stream
+(default priority) and their kernels overlap; --sequential-lines runs them one
+after another on one stream.  Inputs are synthetic: seeded ellipsoid phantoms,
+attenuation maps and Poisson projections at 1e4 peak counts.
+
+The roofline kernel durations come from HIP events around each kernel over K
+steps run one line at a time (right after the timed steps, or the timed steps
+themselves with --sequential-lines): in the concurrent schedule the kernels of
+different lines share the GPU, so their event spans overlap.
 
 Besides the timed steps, the JSON line carries
   * roofline: the dominant kernel's algorithmic HBM bytes over its average
@@ -49,6 +56,9 @@ def parse():
                     help="samples of the batched uneven-instance workload (0: skip)")
     ap.add_argument("--batch-iters", type=int, default=20)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--sequential-lines", action="store_true",
+                    help="time the lines one after another on one stream (default: one stream per line, "
+                         "the lines being independent problems)")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="first run two elementwise kernels of known traffic (4 GiB read + 4 GiB written, with "
                          "16-byte and with 4-byte accesses), to scale HBM counters")
@@ -137,23 +147,57 @@ def main():
                 events[e][2].record()
             C[e], Cn[e] = Cn[e], C[e]
 
+    # The lines are independent problems: by default each runs on its own stream
+    # (default priority), so one line's kernels overlap another's.
+    streams = [torch.cuda.Stream(device=dev) for _ in range(K)]
+
+    def step_concurrent():
+        for e in range(K):
+            with torch.cuda.stream(streams[e]):
+                solvers[e].ratio_step(C[e])
+                solvers[e].update_step(C[e], Cn[e])
+            C[e], Cn[e] = Cn[e], C[e]
+
+    def timed_steps(fn, with_events):
+        """K steps between two events on the current stream (every line's stream
+        waits for the first and is waited for by the second)."""
+        evs = [[[torch.cuda.Event(enable_timing=True) for _ in range(3)] for _ in range(K)] for _ in range(args.steps)]
+        cur = torch.cuda.current_stream(dev)
+        start, stop = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        torch.cuda.synchronize()
+        t_wall = time.perf_counter()
+        start.record(cur)
+        for st in streams:
+            st.wait_event(start)
+        for k in range(args.steps):
+            if with_events:
+                fn(evs[k])
+            else:
+                fn()
+        for st in streams:
+            done = torch.cuda.Event()
+            done.record(st)
+            cur.wait_event(done)
+        stop.record(cur)
+        torch.cuda.synchronize()
+        return start.elapsed_time(stop) / max(args.steps, 1), time.perf_counter() - t_wall, evs
+
+    concurrent = not args.sequential_lines
     for _ in range(args.warmup):
-        step()
+        step_concurrent() if concurrent else step()
     torch.cuda.synchronize()
 
-    evs = [[[torch.cuda.Event(enable_timing=True) for _ in range(3)] for _ in range(K)] for _ in range(args.steps)]
-    start, stop = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-    torch.cuda.synchronize()
-    t_wall = time.perf_counter()
-    start.record()
-    for k in range(args.steps):
-        step(evs[k])
-    stop.record()
-    torch.cuda.synchronize()
-    wall = time.perf_counter() - t_wall
-    total_ms = start.elapsed_time(stop)
-    ms_per_step = total_ms / max(args.steps, 1)
-    log(f"{args.steps} steps: {ms_per_step:.2f} ms/step")
+    if concurrent:
+        ms_per_step, wall, _ = timed_steps(step_concurrent, False)
+        # Kernel durations for the roofline: the same K steps one line at a time
+        # right after the timed region (in the timed steps the lines' kernels
+        # share the GPU, so their event spans overlap).
+        seq_ms_per_step, _, evs = timed_steps(step, True)
+    else:
+        ms_per_step, wall, evs = timed_steps(step, True)
+        seq_ms_per_step = ms_per_step
+    log(f"{args.steps} steps: {ms_per_step:.2f} ms/step ({'one stream per line' if concurrent else 'one stream'}); "
+        f"one line at a time {seq_ms_per_step:.2f} ms/step")
 
     fwd_ms = [evs[k][e][0].elapsed_time(evs[k][e][1]) for k in range(args.steps) for e in range(K)]
     upd_ms = [evs[k][e][1].elapsed_time(evs[k][e][2]) for k in range(args.steps) for e in range(K)]
@@ -188,7 +232,11 @@ def main():
         "config": {"workload": "plain MLEM iteration (beta=0) with attenuation resident in HBM: forward + ratio, "
                                "back-projection + update, all emission lines",
                    "volume": [B, H, W], "angles": A, "elements": K, "refs_per_line": R},
-        "stage_ms": {"forward_ratio_per_line": fwd_avg, "backproject_update_per_line": upd_avg},
+        "schedule": "one stream per line (the lines are independent)" if concurrent else "one stream",
+        "sequential_ms_per_step": seq_ms_per_step,
+        "stage_ms": {"forward_ratio_per_line": fwd_avg, "backproject_update_per_line": upd_avg,
+                     "timing": "kernel events over K steps run one line at a time" +
+                               (", right after the timed steps" if concurrent else " (the timed steps)")},
         "hbm_algorithmic_gbs_per_iteration": iter_bytes / (ms_per_step * 1e-3) / 1e9,
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": measured_traffic(dom_key, N, A),
