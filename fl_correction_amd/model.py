@@ -122,21 +122,52 @@ class XRFModel:
         return bool(self.store)
 
     # ------------------------------------------------------------------
-    def attenuation(self, C, out=None):
-        """Total attenuation (K, A, B, H, W) for concentrations C (K, B, H, W)."""
+    def attenuation(self, C, out=None, streams=2):
+        """Total attenuation (K, A, B, H, W) for concentrations C (K, B, H, W).
+
+        The angles are independent: they are dealt round-robin to `streams`
+        streams (the caller's current stream and default-priority side streams),
+        each with its own scratch maps, so that one angle's kernels overlap
+        another's.  Every angle is computed exactly as with one stream.  The
+        caller's stream waits for the side streams before this returns."""
         C = C.to(self.device, torch.float32).contiguous()
         K, B, H, W = C.shape
         if out is None:
             out = torch.empty((K, self.A, B, H, W), dtype=torch.float32, device=self.device)
-        F, Ax = torch.empty_like(C), torch.empty_like(C)
-        mu_rot = torch.empty((K, B, W, H), dtype=torch.float32, device=self.device)
-        T_rot = torch.empty_like(mu_rot)
+        n_st = max(1, min(int(streams), self.A))
         with torch.cuda.device(self.device):
+            cur = torch.cuda.current_stream(self.device)
+            sts = [cur] + self._side_streams(n_st - 1)
+            # scratch per stream, allocated on the caller's stream (released after it
+            # has waited for the side streams, so no reuse can overtake them)
+            bufs = []
+            for _ in range(n_st):
+                mu_rot = torch.empty((K, B, W, H), dtype=torch.float32, device=self.device)
+                bufs.append((torch.empty_like(C), torch.empty_like(C), mu_rot, torch.empty_like(mu_rot)))
+            ready = torch.cuda.Event()
+            ready.record(cur)
+            for st in sts[1:]:
+                st.wait_event(ready)
             for a in range(self.A):
-                self._angle_maps(C, a, F, Ax, mu_rot, T_rot)
-                _lib.call("fl_xrf_combine", _lib.ptr(T_rot), _lib.ptr(Ax), _lib.ptr(out[0, a]), K, B, H, W, B * H * W,
-                          out.stride(0), self.side, self._st())
+                i = a % n_st
+                with torch.cuda.stream(sts[i]):
+                    F, Ax, mu_rot, T_rot = bufs[i]
+                    self._angle_maps(C, a, F, Ax, mu_rot, T_rot)
+                    _lib.call("fl_xrf_combine", _lib.ptr(T_rot), _lib.ptr(Ax), _lib.ptr(out[0, a]), K, B, H, W,
+                              B * H * W, out.stride(0), self.side, self._st())
+            for st in sts[1:]:
+                done = torch.cuda.Event()
+                done.record(st)
+                cur.wait_event(done)
         return out
+
+    def _side_streams(self, n):
+        """n default-priority streams of this model's device, created once."""
+        if not hasattr(self, "_streams"):
+            self._streams = []
+        while len(self._streams) < n:
+            self._streams.append(torch.cuda.Stream(device=self.device))
+        return self._streams[:n]
 
     def forward(self, C, angle_chunk=16):
         """Pf (K, A, B, W).  Keeps the fluorescence exponents for vjp when storing."""

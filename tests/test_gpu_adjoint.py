@@ -133,3 +133,19 @@ def test_autograd_function_and_attenuation_term_matters():
     assert rel_l2(Pf2.detach().cpu().numpy(), Pf.detach().cpu().numpy()) < 1e-5
     (Pf2 - I * torch.log(Pf2.clamp_min(1e-6))).mean().backward()
     assert rel_l2(Cd2.grad.cpu().numpy(), g_full.cpu().numpy()) > 1e-3
+
+
+@pytest.mark.parametrize("kind", ["cone", "mask"])
+def test_attenuation_streams_agree(kind):
+    """The attenuation maps dealt over 2 or 3 streams equal the one-stream maps bit
+    for bit (every angle is computed the same way), also when the caller's
+    stream has pending work that produced C."""
+    from fl_correction_amd.model import XRFModel
+    C, param, angles, det, mask, em, coef_x, coef_xrf = _model_case(kind, seed=21)
+    model = XRFModel(param, angles, det, em=em)
+    Cd = _dev(C)
+    one = model.attenuation(Cd, streams=1)
+    for n in (2, 3):
+        C2 = (Cd * 2.0) * 0.5  # produced on the current stream just before the call
+        got = model.attenuation(C2, streams=n)
+        assert torch.equal(got, one), n
