@@ -169,8 +169,13 @@ class XRFModel:
             self._streams.append(torch.cuda.Stream(device=self.device))
         return self._streams[:n]
 
-    def forward(self, C, angle_chunk=16):
-        """Pf (K, A, B, W).  Keeps the fluorescence exponents for vjp when storing."""
+    def forward(self, C, angle_chunk=16, streams=2):
+        """Pf (K, A, B, W).  Keeps the fluorescence exponents for vjp when storing.
+
+        Within each chunk of angles the attenuation maps are dealt round-robin to
+        `streams` streams (as in `attenuation`); the chunk's projections then run
+        on the caller's stream after it has waited for them.  Results equal the
+        one-stream ones bit for bit."""
         C = C.to(self.device, torch.float32).contiguous()
         K, B, H, W = C.shape
         if K != self.K:
@@ -180,18 +185,35 @@ class XRFModel:
         Pf = torch.empty((K, self.A, B, W), dtype=torch.float32, device=self.device)
         nA = min(angle_chunk, self.A)
         Atot = torch.empty((K, nA, B, H, W), dtype=torch.float32, device=self.device)
-        F, Ax = torch.empty_like(C), torch.empty_like(C)
-        mu_rot = torch.empty((K, B, W, H), dtype=torch.float32, device=self.device)
-        T_tmp = torch.empty_like(mu_rot)
+        n_st = max(1, min(int(streams), nA))
+        bufs = []
+        for _ in range(n_st):
+            mu_rot = torch.empty((K, B, W, H), dtype=torch.float32, device=self.device)
+            bufs.append((torch.empty_like(C), torch.empty_like(C), mu_rot, torch.empty_like(mu_rot)))
         st = self._st()
         with torch.cuda.device(self.device):
+            cur = torch.cuda.current_stream(self.device)
+            sts = [cur] + self._side_streams(n_st - 1)
             for a0 in range(0, self.A, nA):
                 a1 = min(a0 + nA, self.A)
+                # the side streams start after C is ready and the previous chunk's
+                # projections (which read Atot) are done
+                ready = torch.cuda.Event()
+                ready.record(cur)
+                for sx in sts[1:]:
+                    sx.wait_event(ready)
                 for a in range(a0, a1):
-                    T_rot = self._T_store[a] if store else T_tmp
-                    self._angle_maps(C, a, F, Ax, mu_rot, T_rot)
-                    _lib.call("fl_xrf_combine", _lib.ptr(T_rot), _lib.ptr(Ax), _lib.ptr(Atot[0, a - a0]), K, B, H, W,
-                              B * H * W, Atot.stride(0), self.side, st)
+                    i = (a - a0) % n_st
+                    with torch.cuda.stream(sts[i]):
+                        F, Ax, mu_rot, T_tmp = bufs[i]
+                        T_rot = self._T_store[a] if store else T_tmp
+                        self._angle_maps(C, a, F, Ax, mu_rot, T_rot)
+                        _lib.call("fl_xrf_combine", _lib.ptr(T_rot), _lib.ptr(Ax), _lib.ptr(Atot[0, a - a0]), K, B, H,
+                                  W, B * H * W, Atot.stride(0), self.side, self._st())
+                for sx in sts[1:]:
+                    done = torch.cuda.Event()
+                    done.record(sx)
+                    cur.wait_event(done)
                 for e in range(K):
                     _lib.call("fl_forward_emission_batch", _lib.ptr(Atot[e]), _lib.ptr(C[e]), _lib.ptr(self.em_T[e, a0:]),
                               _lib.ptr(self.theta[a0:]), _lib.ptr(Pf[e, a0]), a1 - a0, B, 1, H, W, st)

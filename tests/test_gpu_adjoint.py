@@ -137,15 +137,18 @@ def test_autograd_function_and_attenuation_term_matters():
 
 @pytest.mark.parametrize("kind", ["cone", "mask"])
 def test_attenuation_streams_agree(kind):
-    """The attenuation maps dealt over 2 or 3 streams equal the one-stream maps bit
-    for bit (every angle is computed the same way), also when the caller's
-    stream has pending work that produced C."""
+    """The attenuation maps and the forward, dealt over 2 or 3 streams, equal the
+    one-stream results bit for bit (every angle is computed the same way), also
+    when the caller's stream has pending work that produced C."""
     from fl_correction_amd.model import XRFModel
     C, param, angles, det, mask, em, coef_x, coef_xrf = _model_case(kind, seed=21)
     model = XRFModel(param, angles, det, em=em)
     Cd = _dev(C)
     one = model.attenuation(Cd, streams=1)
+    Pf1 = model.forward(Cd, angle_chunk=2, streams=1)
     for n in (2, 3):
         C2 = (Cd * 2.0) * 0.5  # produced on the current stream just before the call
         got = model.attenuation(C2, streams=n)
         assert torch.equal(got, one), n
+        # the forward's chunks of 2 angles over 5 angles: a partial last chunk
+        assert torch.equal(model.forward(C2, angle_chunk=2, streams=n), Pf1), n
