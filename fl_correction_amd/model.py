@@ -219,43 +219,72 @@ class XRFModel:
                               _lib.ptr(self.theta[a0:]), _lib.ptr(Pf[e, a0]), a1 - a0, B, 1, H, W, st)
         return Pf
 
-    def vjp(self, C, G, angle_chunk=16):
-        """dL/dC (K, B, H, W) for G = dL/dPf (K, A, B, W)."""
+    def vjp(self, C, G, angle_chunk=16, streams=2):
+        """dL/dC (K, B, H, W) for G = dL/dPf (K, A, B, W).
+
+        Within each chunk of angles the per-angle adjoints are dealt round-robin
+        to `streams` streams, each with its own scratch and its own dL/dC
+        accumulator; the side accumulators are added at the end.  (The sum over
+        angles is then grouped by stream, so results match the one-stream ones
+        to rounding, not bit for bit.)"""
         C = C.to(self.device, torch.float32).contiguous()
         G = G.to(self.device, torch.float32).contiguous()
         K, B, H, W = C.shape
         if tuple(G.shape) != (K, self.A, B, W):
             raise RuntimeError(f"G must be {(K, self.A, B, W)}")
         stored = self._T_store is not None
-        dC = torch.zeros_like(C)
         nA = min(angle_chunk, self.A)
         Atot = torch.empty((K, nA, B, H, W), dtype=torch.float32, device=self.device)
-        F, Ax, gA, gx, dmux, dF = (torch.empty_like(C) for _ in range(6))
-        mu_rot = torch.empty((K, B, W, H), dtype=torch.float32, device=self.device)
-        T_tmp, dT_rot, dmu_rot = torch.empty_like(mu_rot), torch.empty_like(mu_rot), torch.empty_like(mu_rot)
+        n_st = max(1, min(int(streams), nA))
+        acc = [torch.zeros_like(C) for _ in range(n_st)]  # acc[0] is the result
+        bufs = []
+        for _ in range(n_st):
+            F, Ax, gA, gx, dmux, dF = (torch.empty_like(C) for _ in range(6))
+            mu_rot = torch.empty((K, B, W, H), dtype=torch.float32, device=self.device)
+            bufs.append((F, Ax, gA, gx, dmux, dF, mu_rot, torch.empty_like(mu_rot), torch.empty_like(mu_rot),
+                         torch.empty_like(mu_rot)))
         st = self._st()
+        dC = acc[0]
         with torch.cuda.device(self.device):
+            cur = torch.cuda.current_stream(self.device)
+            sts = [cur] + self._side_streams(n_st - 1)
             for a0 in range(0, self.A, nA):
                 a1 = min(a0 + nA, self.A)
+                # side streams start after the inputs are ready and the previous
+                # chunk's projections (which read Atot) are done
+                ready = torch.cuda.Event()
+                ready.record(cur)
+                for sx in sts[1:]:
+                    sx.wait_event(ready)
                 for a in range(a0, a1):
-                    T_rot = self._T_store[a] if stored else T_tmp
-                    self._angle_maps(C, a, F, Ax, mu_rot, T_rot, recompute_T=not stored)
-                    for e in range(K):
-                        _lib.call("fl_sample_grad", _lib.ptr(C[e]), _lib.ptr(self.em_T[e, a:]), _lib.ptr(self.theta[a:]),
-                                  _lib.ptr(G[e, a]), _lib.ptr(gA[e]), 1, B, 1, H, W, 0, st)
-                    _lib.call("fl_xrf_grad_prep", _lib.ptr(T_rot), _lib.ptr(Ax), _lib.ptr(gA), _lib.ptr(dT_rot),
-                              _lib.ptr(gx), _lib.ptr(Atot[0, a - a0]), Atot.stride(0), K, B, H, W, self.side, st)
-                    self._exponent_adjoint(dT_rot, dmu_rot, (K, B, H, W))
-                    _lib.call("fl_incident_grad", _lib.ptr(F), _lib.ptr(self.coef_x[a]), _lib.ptr(Ax), _lib.ptr(gx),
-                              _lib.ptr(dmux), K, B, H, W, st)
-                    _lib.call("fl_mu_adjoint", _lib.ptr(dmu_rot), _lib.ptr(dmux), _lib.ptr(self.coef_xrf[a]),
-                              _lib.ptr(self.coef_x[a]), _lib.ptr(dF), K, K, B, H, W, self.side, st)
-                    c, s = self.trig[a]
-                    _lib.call("fl_rotate_adjoint", _lib.ptr(dF), _lib.ptr(dC), K * B, H, W, c, s, 1, st)
+                    i = (a - a0) % n_st
+                    with torch.cuda.stream(sts[i]):
+                        sa = self._st()
+                        F, Ax, gA, gx, dmux, dF, mu_rot, T_tmp, dT_rot, dmu_rot = bufs[i]
+                        T_rot = self._T_store[a] if stored else T_tmp
+                        self._angle_maps(C, a, F, Ax, mu_rot, T_rot, recompute_T=not stored)
+                        for e in range(K):
+                            _lib.call("fl_sample_grad", _lib.ptr(C[e]), _lib.ptr(self.em_T[e, a:]),
+                                      _lib.ptr(self.theta[a:]), _lib.ptr(G[e, a]), _lib.ptr(gA[e]), 1, B, 1, H, W, 0, sa)
+                        _lib.call("fl_xrf_grad_prep", _lib.ptr(T_rot), _lib.ptr(Ax), _lib.ptr(gA), _lib.ptr(dT_rot),
+                                  _lib.ptr(gx), _lib.ptr(Atot[0, a - a0]), Atot.stride(0), K, B, H, W, self.side, sa)
+                        self._exponent_adjoint(dT_rot, dmu_rot, (K, B, H, W))
+                        _lib.call("fl_incident_grad", _lib.ptr(F), _lib.ptr(self.coef_x[a]), _lib.ptr(Ax),
+                                  _lib.ptr(gx), _lib.ptr(dmux), K, B, H, W, sa)
+                        _lib.call("fl_mu_adjoint", _lib.ptr(dmu_rot), _lib.ptr(dmux), _lib.ptr(self.coef_xrf[a]),
+                                  _lib.ptr(self.coef_x[a]), _lib.ptr(dF), K, K, B, H, W, self.side, sa)
+                        c, s = self.trig[a]
+                        _lib.call("fl_rotate_adjoint", _lib.ptr(dF), _lib.ptr(acc[i]), K * B, H, W, c, s, 1, sa)
+                for sx in sts[1:]:
+                    done = torch.cuda.Event()
+                    done.record(sx)
+                    cur.wait_event(done)
                 for e in range(K):
                     _lib.call("fl_backward_emission_accumulate", _lib.ptr(Atot[e]), _lib.ptr(G[e, a0]),
                               _lib.ptr(self.em_T[e, a0:]), _lib.ptr(self.theta[a0:]), _lib.ptr(dC[e]), a1 - a0, B, 1,
                               H, W, st)
+            for extra in acc[1:]:
+                dC.add_(extra)  # on the caller's stream, after it has waited for the side streams
         return dC
 
     def release(self):

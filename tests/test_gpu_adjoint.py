@@ -139,7 +139,8 @@ def test_autograd_function_and_attenuation_term_matters():
 def test_attenuation_streams_agree(kind):
     """The attenuation maps and the forward, dealt over 2 or 3 streams, equal the
     one-stream results bit for bit (every angle is computed the same way), also
-    when the caller's stream has pending work that produced C."""
+    when the caller's stream has pending work that produced C; the vjp equals
+    the one-stream vjp to rounding (its sum over angles is grouped by stream)."""
     from fl_correction_amd.model import XRFModel
     C, param, angles, det, mask, em, coef_x, coef_xrf = _model_case(kind, seed=21)
     model = XRFModel(param, angles, det, em=em)
@@ -152,3 +153,10 @@ def test_attenuation_streams_agree(kind):
         assert torch.equal(got, one), n
         # the forward's chunks of 2 angles over 5 angles: a partial last chunk
         assert torch.equal(model.forward(C2, angle_chunk=2, streams=n), Pf1), n
+    # vjp: the sum over angles is grouped by stream, so equal to rounding
+    G = torch.randn_like(Pf1)
+    model.forward(Cd, angle_chunk=2, streams=1)
+    g1 = model.vjp(Cd, G, angle_chunk=2, streams=1).cpu().numpy()
+    for n in (2, 3):
+        gn = model.vjp(Cd, G, angle_chunk=2, streams=n).cpu().numpy()
+        assert rel_l2(gn, g1) <= 1e-6, n
