@@ -55,20 +55,21 @@ def make_param(elem_type, elem_compound, cs_x, cs_line, rho=1.0, pix=1.0e-4, em_
 
 
 def extract_em_cs_elem(param, elem, use_ref=False):
-    """Emission cross-sections of one element, (n,) or (n, n_ref) (FL/atten_util.py:63-76)."""
-    em_cs_dict = param["em_cs"]
-    em_cs = em_cs_dict[elem]
-    n = len(em_cs)
-    if use_ref:
-        try:
-            n_ref = em_cs_dict["n_ref"]
-            em_cs = np.zeros((n, n_ref))
-            for i in range(n_ref):
-                em_cs[:, i] = em_cs_dict[f"{elem}_ref{i + 1}"]
-        except Exception as err:
-            print(err)
-            print(f"fail to extract em_cs for {elem} using reference")
-    return em_cs
+    """Emission cross-sections of one element (FL/atten_util.py:63-76): param['em_cs'][elem]
+    as given, or with use_ref an (n, n_ref) array whose column i holds
+    param['em_cs'][f'{elem}_ref{i+1}'] (n = len(param['em_cs'][elem]), n_ref =
+    param['em_cs']['n_ref']).  A missing key raises KeyError naming it; the reference
+    prints the error and returns a partly filled array instead."""
+    table = param["em_cs"]
+    base = table[elem]
+    if not use_ref:
+        return base
+    wanted = ["n_ref"] + [f"{elem}_ref{i + 1}" for i in range(int(table.get("n_ref", 0)))]
+    missing = [k for k in wanted if k not in table]
+    if missing:
+        raise KeyError(f"em_cs of {elem} with references needs param['em_cs'] keys {missing}")
+    cols = [np.broadcast_to(np.asarray(table[k], np.float64), (len(base),)) for k in wanted[1:]]
+    return np.stack(cols, axis=1) if cols else np.zeros((len(base), 0))
 
 
 def update_dict_use_ref_at_angle(dict_use_ref, current_angle):
@@ -131,10 +132,8 @@ def coef_xrf(param_angle_energy):
     return out.astype(np.float32)
 
 
-def cal_total_atten_at_angles(atten_xrf, atten_xray):
-    """Per-element product of the two attenuation stacks (FL/atten_util.py:41-58 and
-    FL/core.py:84-101, which read and write them as per-angle TIFF files; here they are
-    device tensors of shape (K, A, B, H, W))."""
-    if atten_xrf.shape != atten_xray.shape:
-        raise RuntimeError("attenuation stacks differ in shape")
-    return atten_xrf * atten_xray
+def cal_total_atten_at_angles(fsave, param):
+    """Total attenuation files from the fluorescence and incident file sets
+    (FL/atten_util.py:41-58, the same function as FL/core.py:84-101)."""
+    from .core import cal_total_atten_at_angles as total
+    return total(fsave, param)

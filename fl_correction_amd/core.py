@@ -2,21 +2,24 @@
 
 The reference computes each angle's maps, copies them to the host and writes
 them as TIFF files (FL/core.py:19-101, FL/file_io.py:69-126), then reads them
-back for MLEM.  Here the maps stay in device memory: each driver returns one
+back for MLEM.  The drivers here do the same when given `fsave` (the reference's
+default './Angle_prj' included), and always also return the maps as one device
 tensor of shape (K, A, B, H, W) (element, angle, slice, row, col), i.e. one
-(A, B, H, W) attenuation stack per emission line, the layout
-``mlem_cuda_batch`` and the projector consume.  ``cal_atten_at_angles`` fuses
-the three reference drivers (fluorescence path, incident beam, product) into one
-pass per angle.
+(A, B, H, W) attenuation stack per emission line, the layout ``mlem_cuda_batch``
+and the projector consume.  ``cal_atten_at_angles`` fuses the three reference
+drivers (fluorescence path, incident beam, product) into one pass per angle
+and writes no files.
 """
 from __future__ import annotations
 
+import glob
 import math
+import os
 
 import numpy as np
 import torch
 
-from . import _lib
+from . import _lib, file_io
 from .atten_util import coef_xray, coef_xrf, extract_single_value_param, update_dict_use_ref_at_angle
 from .cuda_lib.atten_cuda import forward_emission_batch
 from .torch_functions import _side, xrf_exponent_rot
@@ -25,7 +28,7 @@ from .torch_functions import _side, xrf_exponent_rot
 def angle_entries(angle_deg):
     """float32 rotation-matrix entries for an angle in degrees, as the reference builds them:
     theta = torch.tensor(angle/180*pi) (float32, FL/core.py:35), then math.cos/sin of it
-    rounded to float32 (FL/torch_image_util.py:401-409)."""
+    rounded to float32 (FL/torch_image_util.py:77-85)."""
     th32 = float(np.float32(angle_deg / 180.0 * np.pi))
     return float(np.float32(math.cos(th32))), float(np.float32(math.sin(th32)))
 
@@ -56,10 +59,20 @@ def _rotate(frac, angle_deg, buf):
     return buf
 
 
+def _write_maps(out, elem_type, fsave, mode):
+    """Every angle's (B, H, W) map of each element as the reference names it
+    (write_attenuation_fl / _xray / write_attenuation, FL/file_io.py:69-126)."""
+    if fsave is None:
+        return
+    file_io.write_attenuation_stack(out, list(elem_type), fsave, mode)
+
+
 def cal_xrf_atten_at_angles(frac_4D, param, angle_list, detector_mask, detector_offset_angle=0, position_det="r",
-                            dict_use_ref={}, fsave=None, out=None):
+                            dict_use_ref={}, fsave="./Angle_prj", out=None):
     """Fluorescence-path attenuation at every angle (FL/core.py:19-52): rotate frac by the
-    angle, exp(-(mask (*) mu_line)) per element.  Returns (K, A, B, H, W)."""
+    angle, exp(-(mask (*) mu_line)) per element.  As the reference, writes
+    atten_fl_{elem}_prj_{id:04d}.tiff into `fsave` (default './Angle_prj'); fsave=None
+    keeps the maps on the device only.  Returns the device stack (K, A, B, H, W)."""
     frac, angles = _prep(frac_4D, angle_list, dict_use_ref, detector_offset_angle)
     K, B, H, W = frac.shape
     A = len(angles)
@@ -76,11 +89,14 @@ def cal_xrf_atten_at_angles(frac_4D, param, angle_list, detector_mask, detector_
             T = xrf_exponent_rot(rot, coef_xrf(p), detector_mask, side, work)
             _lib.call("fl_xrf_combine", _lib.ptr(T), None, _lib.ptr(out[0, a]), K, B, H, W, 0, out.stride(0), side,
                       _lib.stream_of(dev))
+    _write_maps(out, param["elem_type"], fsave, "fl")
     return out
 
 
-def cal_xray_atten_at_angles(frac_4D, param, angle_list, dict_use_ref={}, fsave=None, out=None):
-    """Incident-beam attenuation at every angle (FL/core.py:55-81).  Returns (K, A, B, H, W)."""
+def cal_xray_atten_at_angles(frac_4D, param, angle_list, dict_use_ref={}, fsave="./Angle_prj", out=None):
+    """Incident-beam attenuation at every angle (FL/core.py:55-81).  Writes
+    atten_incident_xray_{elem}_prj_{id:04d}.tiff into `fsave` unless it is None.
+    Returns the device stack (K, A, B, H, W)."""
     frac, angles = _prep(frac_4D, angle_list, dict_use_ref)
     K, B, H, W = frac.shape
     A = len(angles)
@@ -94,11 +110,29 @@ def cal_xray_atten_at_angles(frac_4D, param, angle_list, dict_use_ref={}, fsave=
             coef = torch.as_tensor(coef_xray(p), device=dev)
             _lib.call("fl_incident_atten", _lib.ptr(rot), _lib.ptr(coef), _lib.ptr(out[0, a]), K, B, H, W, B * H * W,
                       out.stride(0), _lib.stream_of(dev))
+    _write_maps(out, param["elem_type"], fsave, "xray")
     return out
 
 
-def cal_total_atten_at_angles(atten_xrf, atten_xray):
-    """Product of the two stacks (FL/core.py:84-101 without the file round trip)."""
+def cal_total_atten_at_angles(fsave, param):
+    """Total attenuation files from the two per-path file sets (FL/core.py:84-101): the
+    angle count is the number of atten_fl_* files over the element count; for every
+    element and angle, atten_{elem}_prj_{id:04d}.tiff = fl map * incident map (a float32
+    product: both maps are float32, so the reference's float64 product rounds to it)."""
+    elem_type = list(param["elem_type"])
+    n_angle = len(glob.glob(os.path.join(fsave, "atten_fl*"))) // len(elem_type)
+    for elem in elem_type:
+        for a in range(n_angle):
+            fl = file_io.read_attenuation_fl(a, fsave, elem)
+            xr = file_io.read_attenuation_xray(a, fsave, elem)
+            if fl.shape != xr.shape:
+                raise RuntimeError(f"{elem} angle {a}: fluorescence map {fl.shape} and incident map {xr.shape} differ")
+            file_io.write_attenuation(elem, fl * xr, a, fsave)
+
+
+def cal_total_atten_stack(atten_xrf, atten_xray):
+    """Product of the two device stacks (K, A, B, H, W): cal_total_atten_at_angles
+    without the file round trip."""
     if atten_xrf.shape != atten_xray.shape:
         raise RuntimeError("attenuation stacks differ in shape")
     return atten_xrf * atten_xray

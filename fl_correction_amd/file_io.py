@@ -100,10 +100,9 @@ def write_attenuation_stack(atten, elem_list, file_path="./Angle_prj", mode="all
     if len(elem_list) != K:
         raise ValueError("one element name per map")
     for k in range(K):
+        maps = atten[k].detach().cpu().numpy() if hasattr(atten, "detach") else np.asarray(atten[k])
         for a in range(A):
-            m = atten[k, a]
-            m = m.detach().cpu().numpy() if hasattr(m, "detach") else np.asarray(m)
-            _write(elem_list[k], m, a, file_path, mode)
+            _write(elem_list[k], maps[a], a, file_path, mode)
 
 
 def read_attenuation_stack(n_angle, elem_list, file_path="./Angle_prj", mode="all", device=None):

@@ -24,14 +24,14 @@ def _angle_value(theta):
 
 def rotation_entries(theta):
     """float32 (cos, sin) entries of the affine matrix, computed as the reference does:
-    math.cos of the (float32) angle, rounded to float32 (torch_image_util.py:401-409)."""
+    math.cos of the (float32) angle, rounded to float32 (FL/torch_image_util.py:77-85)."""
     th = _angle_value(theta)
     return float(np.float32(math.cos(th))), float(np.float32(math.sin(th)))
 
 
 def torch_rotate_image_nd(x, theta, enable_crop=True, align_corners=False, mode="bilinear", padding_mode="zeros"):
     """Rotate (..., H, W) by theta radians (positive = counter-clockwise) about the centre
-    (FL/torch_image_util.py:337-438)."""
+    (FL/torch_image_util.py:13-114)."""
     if not (enable_crop and not align_corners and mode == "bilinear" and padding_mode == "zeros"):
         raise NotImplementedError("only enable_crop=True, align_corners=False, bilinear, zeros are implemented")
     if not isinstance(x, torch.Tensor) or not x.is_cuda:
@@ -51,30 +51,36 @@ def torch_rotate_image_nd(x, theta, enable_crop=True, align_corners=False, mode=
 
 
 def torch_rot90_3D(img_raw, ax=0, mode="c-clock"):
-    """90-degree rotation of a 3-D tensor about axis `ax` (FL/torch_image_util.py:441-471):
-    a transpose and a flip, as in the reference."""
-    img_r = img_raw.clone()
-    if ax == 0:
-        img_r = torch.transpose(img_r, 1, 2)
-        if mode == "clock":
-            img_r = img_r.flip(dims=(2,))
-        elif mode == "c-clock":
-            img_r = img_r.flip(dims=(1,))
-    if ax == 1:
-        img_r = torch.transpose(img_r, 0, 2)
-        if mode == "clock":
-            img_r = img_r.flip(dims=(2,))
-        elif mode == "c-clock":
-            img_r = img_r.flip(dims=(0,))
-    if ax == 2:
-        img_r = torch.transpose(img_r, 0, 1)
-        if mode == "clock":
-            img_r = img_r.flip(dims=(0,))
-        elif mode == "c-clock":
-            img_r = img_r.flip(dims=(1,))
-    return img_r
+    """Quarter turn of a 3-D tensor about axis `ax` (FL/torch_image_util.py:117-147).
+
+    With the other two axes (p, q) in increasing order, 'c-clock' and 'clock' are
+    torch.rot90 by +1 and -1 quarter turns in the plane (p, q), except about axis 2,
+    where the reference turns the other way (its positive direction runs left to
+    right).  Any other mode only swaps p and q.  The index identities, for
+    img (B, H, W) about axis 0:
+        clock    out[b, i, j] = img[b, H-1-j, i]
+        c-clock  out[b, i, j] = img[b, j, W-1-i]
+    """
+    if ax not in _ROT90_PLANE:
+        return img_raw.clone()
+    plane = _ROT90_PLANE[ax]
+    turns = _ROT90_TURNS.get(mode)
+    if turns is None:
+        return img_raw.transpose(*plane).clone()
+    return torch.rot90(img_raw, turns if ax != 2 else -turns, plane)
+
+
+# the plane each axis turns, and the quarter turns of each mode
+_ROT90_PLANE = {0: (1, 2), 1: (0, 2), 2: (0, 1)}
+_ROT90_TURNS = {"c-clock": 1, "clock": -1}
 
 
 def torch_rot90_4D(img4D, ax=0, mode="c-clock"):
-    """torch_rot90_3D applied to each element of a 4-D stack (FL/torch_image_util.py:474-479)."""
-    return torch.stack([torch_rot90_3D(img4D[i], ax, mode) for i in range(img4D.shape[0])])
+    """torch_rot90_3D of every element of a 4-D stack at once (FL/torch_image_util.py:150-155)."""
+    if ax not in _ROT90_PLANE:
+        return img4D.clone()
+    plane = tuple(p + 1 for p in _ROT90_PLANE[ax])
+    turns = _ROT90_TURNS.get(mode)
+    if turns is None:
+        return img4D.transpose(*plane).clone()
+    return torch.rot90(img4D, turns if ax != 2 else -turns, plane)

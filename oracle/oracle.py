@@ -315,9 +315,13 @@ def rotate_f64(x, theta):
 
 
 def g_mask_loops(M_shape, alfa, N):
-    """Scalar-loop restatement of ``g_mask`` (detector_mask.py:109-151)."""
-    tan_alfa = np.tan(alfa)
-    cos_alfa = np.cos(alfa)
+    """Scalar-loop restatement of ``g_mask`` (detector_mask.py:109-151).  The reference
+    is numba-compiled: tan and cos of the float32 angle are float32, and every product
+    with the int64 loop indices is float64 (numba's int64 x float32 -> float64), so the
+    loops here take the float32 values as float64 numbers.  Pinned against the wedge
+    areas of each cell in tests/test_mask_pin.py."""
+    tan_alfa = float(np.tan(np.float32(alfa)))
+    cos_alfa = float(np.cos(np.float32(alfa)))
     M1 = np.zeros((M_shape[0], M_shape[1]))
     for I in range(M_shape[0]):
         for J in range(M_shape[1]):
@@ -344,7 +348,9 @@ def g_mask_loops(M_shape, alfa, N):
 
 def generate_detector_mask(alfa0, theta0, leng0):
     """Restatement of ``generate_detector_mask`` (detector_mask.py:22-106) with
-    ``g_radial_mask_approximate`` (:179-186).  Returns (slice, row, col) = (Mz, Mx, My)."""
+    ``g_radial_mask_approximate`` (:179-186).  Returns (slice, row, col) = (Mz, Mx, My).
+    Pinned by the wedge-area construction, the per-depth sums and the prefix property
+    (tests/test_mask_pin.py)."""
     alfa = np.float32(alfa0) / 2 / 180 * np.pi
     theta = np.float32(theta0) / 2 / 180 * np.pi
     N1_0 = np.int16(np.ceil(leng0 * np.tan(alfa)))
