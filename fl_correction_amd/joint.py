@@ -95,13 +95,14 @@ def self_consistent_mlem(model: XRFModel, I, C_init, n_iter=50, recompute_every=
     return C
 
 
-def joint_reconstruct(model: XRFModel, I, C0, n_steps=100, lr=None, eps=1e-6, callback=None):
+def joint_reconstruct(model: XRFModel, I, C0, n_steps=100, lr=None, eps=1e-6, callback=None, streams=2):
     """Adam on all element maps with a non-negativity projection.
 
     I (K, A, B, W) measured projections, C0 (K, B, H, W) start.  lr defaults to
     2% of the start's mean value.  Returns (C, history) with history a list of
     (step, loss, data residual) before each update and after the last one.
-    callback(step, C, loss) is called after every step if given."""
+    callback(step, C, loss) is called after every step if given.  `streams` is
+    the model's stream count for every forward and vjp."""
     I = I.to(model.device, torch.float32).contiguous()
     C = C0.to(model.device, torch.float32).clone().contiguous().requires_grad_(True)
     if lr is None:
@@ -110,7 +111,7 @@ def joint_reconstruct(model: XRFModel, I, C0, n_steps=100, lr=None, eps=1e-6, ca
     history = []
     for step in range(int(n_steps) + 1):
         opt.zero_grad(set_to_none=True)
-        Pf = XRFModelFunction.apply(C, model)
+        Pf = XRFModelFunction.apply(C, model, streams)
         loss = poisson_loss(Pf, I, eps)
         history.append((step, float(loss.detach()), data_residual(Pf.detach(), I)))
         if step == n_steps:

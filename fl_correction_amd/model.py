@@ -114,12 +114,26 @@ class XRFModel:
                       self.side, st)
             self._exponent(mu_rot, T_rot, (K, B, H, W))
 
-    def _use_store(self, C):
+    def _use_store(self, C, streams=2, angle_chunk=16):
+        """Whether to keep the exponents: with 'auto', when they fit memory_fraction of the
+        free memory left after the vjp's own scratch (per stream ten maps of C's size and
+        one accumulator, plus one chunk of attenuation maps)."""
         if self.store == "auto":
-            need = 4 * self.K * self.A * C[0].numel()
+            vol = 4 * C.numel()
+            need = self.A * vol
+            n_st = max(1, min(int(streams), self.A))
+            scratch = n_st * 11 * vol + min(angle_chunk, self.A) * vol
             free, _ = torch.cuda.mem_get_info(self.device)
-            return need <= self.memory_fraction * free
+            return need <= self.memory_fraction * max(free - scratch, 0)
         return bool(self.store)
+
+    @staticmethod
+    def _join(cur, side):
+        """Make the caller's stream `cur` wait for everything queued on the side streams."""
+        for st in side:
+            done = torch.cuda.Event()
+            done.record(st)
+            cur.wait_event(done)
 
     # ------------------------------------------------------------------
     def attenuation(self, C, out=None, streams=2):
@@ -148,17 +162,18 @@ class XRFModel:
             ready.record(cur)
             for st in sts[1:]:
                 st.wait_event(ready)
-            for a in range(self.A):
-                i = a % n_st
-                with torch.cuda.stream(sts[i]):
-                    F, Ax, mu_rot, T_rot = bufs[i]
-                    self._angle_maps(C, a, F, Ax, mu_rot, T_rot)
-                    _lib.call("fl_xrf_combine", _lib.ptr(T_rot), _lib.ptr(Ax), _lib.ptr(out[0, a]), K, B, H, W,
-                              B * H * W, out.stride(0), self.side, self._st())
-            for st in sts[1:]:
-                done = torch.cuda.Event()
-                done.record(st)
-                cur.wait_event(done)
+            try:
+                for a in range(self.A):
+                    i = a % n_st
+                    with torch.cuda.stream(sts[i]):
+                        F, Ax, mu_rot, T_rot = bufs[i]
+                        self._angle_maps(C, a, F, Ax, mu_rot, T_rot)
+                        _lib.call("fl_xrf_combine", _lib.ptr(T_rot), _lib.ptr(Ax), _lib.ptr(out[0, a]), K, B, H, W,
+                                  B * H * W, out.stride(0), self.side, self._st())
+            finally:
+                # also on an error: the scratch returns to the caller's stream's pool only
+                # after the side streams are done with it
+                self._join(cur, sts[1:])
         return out
 
     def _side_streams(self, n):
@@ -180,7 +195,8 @@ class XRFModel:
         K, B, H, W = C.shape
         if K != self.K:
             raise RuntimeError(f"C has {K} element maps, the model {self.K}")
-        store = self._use_store(C)
+        self._T_store = None
+        store = self._use_store(C, streams, angle_chunk)
         self._T_store = torch.empty((self.A, K, B, W, H), dtype=torch.float32, device=self.device) if store else None
         Pf = torch.empty((K, self.A, B, W), dtype=torch.float32, device=self.device)
         nA = min(angle_chunk, self.A)
@@ -202,18 +218,17 @@ class XRFModel:
                 ready.record(cur)
                 for sx in sts[1:]:
                     sx.wait_event(ready)
-                for a in range(a0, a1):
-                    i = (a - a0) % n_st
-                    with torch.cuda.stream(sts[i]):
-                        F, Ax, mu_rot, T_tmp = bufs[i]
-                        T_rot = self._T_store[a] if store else T_tmp
-                        self._angle_maps(C, a, F, Ax, mu_rot, T_rot)
-                        _lib.call("fl_xrf_combine", _lib.ptr(T_rot), _lib.ptr(Ax), _lib.ptr(Atot[0, a - a0]), K, B, H,
-                                  W, B * H * W, Atot.stride(0), self.side, self._st())
-                for sx in sts[1:]:
-                    done = torch.cuda.Event()
-                    done.record(sx)
-                    cur.wait_event(done)
+                try:
+                    for a in range(a0, a1):
+                        i = (a - a0) % n_st
+                        with torch.cuda.stream(sts[i]):
+                            F, Ax, mu_rot, T_tmp = bufs[i]
+                            T_rot = self._T_store[a] if store else T_tmp
+                            self._angle_maps(C, a, F, Ax, mu_rot, T_rot)
+                            _lib.call("fl_xrf_combine", _lib.ptr(T_rot), _lib.ptr(Ax), _lib.ptr(Atot[0, a - a0]), K,
+                                      B, H, W, B * H * W, Atot.stride(0), self.side, self._st())
+                finally:
+                    self._join(cur, sts[1:])
                 for e in range(K):
                     _lib.call("fl_forward_emission_batch", _lib.ptr(Atot[e]), _lib.ptr(C[e]), _lib.ptr(self.em_T[e, a0:]),
                               _lib.ptr(self.theta[a0:]), _lib.ptr(Pf[e, a0]), a1 - a0, B, 1, H, W, st)
@@ -256,29 +271,10 @@ class XRFModel:
                 ready.record(cur)
                 for sx in sts[1:]:
                     sx.wait_event(ready)
-                for a in range(a0, a1):
-                    i = (a - a0) % n_st
-                    with torch.cuda.stream(sts[i]):
-                        sa = self._st()
-                        F, Ax, gA, gx, dmux, dF, mu_rot, T_tmp, dT_rot, dmu_rot = bufs[i]
-                        T_rot = self._T_store[a] if stored else T_tmp
-                        self._angle_maps(C, a, F, Ax, mu_rot, T_rot, recompute_T=not stored)
-                        for e in range(K):
-                            _lib.call("fl_sample_grad", _lib.ptr(C[e]), _lib.ptr(self.em_T[e, a:]),
-                                      _lib.ptr(self.theta[a:]), _lib.ptr(G[e, a]), _lib.ptr(gA[e]), 1, B, 1, H, W, 0, sa)
-                        _lib.call("fl_xrf_grad_prep", _lib.ptr(T_rot), _lib.ptr(Ax), _lib.ptr(gA), _lib.ptr(dT_rot),
-                                  _lib.ptr(gx), _lib.ptr(Atot[0, a - a0]), Atot.stride(0), K, B, H, W, self.side, sa)
-                        self._exponent_adjoint(dT_rot, dmu_rot, (K, B, H, W))
-                        _lib.call("fl_incident_grad", _lib.ptr(F), _lib.ptr(self.coef_x[a]), _lib.ptr(Ax),
-                                  _lib.ptr(gx), _lib.ptr(dmux), K, B, H, W, sa)
-                        _lib.call("fl_mu_adjoint", _lib.ptr(dmu_rot), _lib.ptr(dmux), _lib.ptr(self.coef_xrf[a]),
-                                  _lib.ptr(self.coef_x[a]), _lib.ptr(dF), K, K, B, H, W, self.side, sa)
-                        c, s = self.trig[a]
-                        _lib.call("fl_rotate_adjoint", _lib.ptr(dF), _lib.ptr(acc[i]), K * B, H, W, c, s, 1, sa)
-                for sx in sts[1:]:
-                    done = torch.cuda.Event()
-                    done.record(sx)
-                    cur.wait_event(done)
+                try:
+                    self._vjp_angles(C, G, a0, a1, n_st, sts, bufs, acc, Atot, stored)
+                finally:
+                    self._join(cur, sts[1:])
                 for e in range(K):
                     _lib.call("fl_backward_emission_accumulate", _lib.ptr(Atot[e]), _lib.ptr(G[e, a0]),
                               _lib.ptr(self.em_T[e, a0:]), _lib.ptr(self.theta[a0:]), _lib.ptr(dC[e]), a1 - a0, B, 1,
@@ -287,24 +283,49 @@ class XRFModel:
                 dC.add_(extra)  # on the caller's stream, after it has waited for the side streams
         return dC
 
+    def _vjp_angles(self, C, G, a0, a1, n_st, sts, bufs, acc, Atot, stored):
+        """The per-angle adjoints of angles a0..a1-1, dealt round-robin to the streams."""
+        K, B, H, W = C.shape
+        for a in range(a0, a1):
+            i = (a - a0) % n_st
+            with torch.cuda.stream(sts[i]):
+                sa = self._st()
+                F, Ax, gA, gx, dmux, dF, mu_rot, T_tmp, dT_rot, dmu_rot = bufs[i]
+                T_rot = self._T_store[a] if stored else T_tmp
+                self._angle_maps(C, a, F, Ax, mu_rot, T_rot, recompute_T=not stored)
+                for e in range(K):
+                    _lib.call("fl_sample_grad", _lib.ptr(C[e]), _lib.ptr(self.em_T[e, a:]),
+                              _lib.ptr(self.theta[a:]), _lib.ptr(G[e, a]), _lib.ptr(gA[e]), 1, B, 1, H, W, 0, sa)
+                _lib.call("fl_xrf_grad_prep", _lib.ptr(T_rot), _lib.ptr(Ax), _lib.ptr(gA), _lib.ptr(dT_rot),
+                          _lib.ptr(gx), _lib.ptr(Atot[0, a - a0]), Atot.stride(0), K, B, H, W, self.side, sa)
+                self._exponent_adjoint(dT_rot, dmu_rot, (K, B, H, W))
+                _lib.call("fl_incident_grad", _lib.ptr(F), _lib.ptr(self.coef_x[a]), _lib.ptr(Ax),
+                          _lib.ptr(gx), _lib.ptr(dmux), K, B, H, W, sa)
+                _lib.call("fl_mu_adjoint", _lib.ptr(dmu_rot), _lib.ptr(dmux), _lib.ptr(self.coef_xrf[a]),
+                          _lib.ptr(self.coef_x[a]), _lib.ptr(dF), K, K, B, H, W, self.side, sa)
+                c, s = self.trig[a]
+                _lib.call("fl_rotate_adjoint", _lib.ptr(dF), _lib.ptr(acc[i]), K * B, H, W, c, s, 1, sa)
+
     def release(self):
         self._T_store = None
 
 
 class XRFModelFunction(torch.autograd.Function):
-    """Pf = model(C) with the hand-written adjoint as its backward."""
+    """Pf = model(C) with the hand-written adjoint as its backward.  `streams` is passed
+    to the model's forward and vjp (1 keeps every launch on the caller's stream)."""
 
     @staticmethod
-    def forward(ctx, C, model):
+    def forward(ctx, C, model, streams=2):
         ctx.model = model
+        ctx.streams = streams
         ctx.save_for_backward(C)
-        return model.forward(C.detach())
+        return model.forward(C.detach(), streams=streams)
 
     @staticmethod
     def backward(ctx, G):
         (C,) = ctx.saved_tensors
-        return ctx.model.vjp(C.detach(), G), None
+        return ctx.model.vjp(C.detach(), G, streams=ctx.streams), None, None
 
 
-def xrf_forward_autograd(C, model):
-    return XRFModelFunction.apply(C, model)
+def xrf_forward_autograd(C, model, streams=2):
+    return XRFModelFunction.apply(C, model, streams)

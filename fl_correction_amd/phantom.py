@@ -16,17 +16,20 @@ def rng_for(seed):
     return np.random.Generator(np.random.PCG64(seed))
 
 
-def ellipsoid_phantom(shape, n_elem, n_ell=(6, 12), semi_axes=(4.0, 16.0), conc=(0.1, 1.0), seed=0):
+def ellipsoid_phantom(shape, n_elem, n_ell=(6, 12), semi_axes=(4.0, 16.0), conc=(0.1, 1.0), seed=0, box=None):
     """Concentration maps (n_elem, B, H, W) float32.
 
     shape: (B, H, W) = (slices, rows, cols).  n_ell: inclusive range of the number
     of ellipsoids per element.  semi_axes: range of each semi-axis in voxels.
+    box: ((b0, b1), (h0, h1), (w0, w1)) evaluates only that sub-box of the volume
+    (the same draws, so the values equal the full phantom's crop).
     """
     B, H, W = shape
     rng = rng_for(seed)
-    out = np.zeros((n_elem, B, H, W), np.float32)
-    zz, yy, xx = np.meshgrid(np.arange(B, dtype=np.float32), np.arange(H, dtype=np.float32),
-                             np.arange(W, dtype=np.float32), indexing="ij")
+    (b0, b1), (h0, h1), (w0, w1) = box if box is not None else ((0, B), (0, H), (0, W))
+    out = np.zeros((n_elem, b1 - b0, h1 - h0, w1 - w0), np.float32)
+    zz, yy, xx = np.meshgrid(np.arange(b0, b1, dtype=np.float32), np.arange(h0, h1, dtype=np.float32),
+                             np.arange(w0, w1, dtype=np.float32), indexing="ij")
     cy, cx = (H - 1) / 2.0, (W - 1) / 2.0
     r_cyl = min(H, W) / 2.0
     for e in range(n_elem):

@@ -25,7 +25,7 @@
  *        :390-446, huber_grad_kernel :495-534, huber_grad_batch_kernel :564-616
  *   FL/cuda_lib/atten_cuda.cpp         mlem_cuda :244-281, mlem_cuda_batch :283-345
  *   FL/torch_functions.py              torch_xray_atten_3D :83-106 (incident product)
- *   FL/torch_image_util.py             torch_rotate_image_nd :337-438 (grid_sample rotation)
+ *   FL/torch_image_util.py             torch_rotate_image_nd :13-114 (grid_sample rotation)
  */
 #include <math.h>
 #include <stdint.h>
@@ -510,7 +510,7 @@ void or_incident_f64(const double *mux, double *out, int N, int H, int W)
 }
 
 /* ------------------------------------------------------------------------- */
-/* A5: in-plane rotation, FL/torch_image_util.py:337-438 with enable_crop,    */
+/* A5: in-plane rotation, FL/torch_image_util.py:13-114 with enable_crop,     */
 /* align_corners=False, bilinear, zero padding: affine_grid with              */
 /* [[cos,-sin,0],[sin,cos,0]] followed by grid_sample.  Double arithmetic.    */
 /* ------------------------------------------------------------------------- */
@@ -559,13 +559,17 @@ static inline int64_t floordiv64(int64_t a, int64_t b)
     return (r != 0 && ((r < 0) != (b < 0))) ? q - 1 : q;
 }
 
-void or_cone_march_f64(const double *mu, double *out, int Z, int X, int Y, const int *pz,
-                       const int *py, const int *qd, const double *wgt, int ndir)
+/* Outputs z0 <= z < z1, y0 <= y < y1 only (all of mu is read): out is          */
+/* (z1-z0, X, y1-y0).                                                          */
+void or_cone_march_box_f64(const double *mu, double *out, int Z, int X, int Y, int z0s, int z1s,
+                           int y0s, int y1s, const int *pz, const int *py, const int *qd,
+                           const double *wgt, int ndir)
 {
-#pragma omp parallel for collapse(2) schedule(static)
-    for (int z = 0; z < Z; ++z)
+    const int Yo = y1s - y0s;
+#pragma omp parallel for collapse(2) schedule(dynamic, 4)
+    for (int z = z0s; z < z1s; ++z)
         for (int x = 0; x < X; ++x)
-            for (int y = 0; y < Y; ++y) {
+            for (int y = y0s; y < y1s; ++y) {
                 double acc = 0.0;
                 for (int d = 0; d < ndir; ++d) {
                     double sd = 0.0;
@@ -585,8 +589,14 @@ void or_cone_march_f64(const double *mu, double *out, int Z, int X, int Y, const
                     }
                     acc += wgt[d] * sd;
                 }
-                out[((int64_t)z * X + x) * Y + y] = acc;
+                out[((int64_t)(z - z0s) * X + x) * Yo + (y - y0s)] = acc;
             }
+}
+
+void or_cone_march_f64(const double *mu, double *out, int Z, int X, int Y, const int *pz,
+                       const int *py, const int *qd, const double *wgt, int ndir)
+{
+    or_cone_march_box_f64(mu, out, Z, X, Y, 0, Z, 0, Y, pz, py, qd, wgt, ndir);
 }
 
 /* ------------------------------------------------------------------------- */

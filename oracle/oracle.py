@@ -251,7 +251,7 @@ def incident_f64(mux):
 
 
 def rot90_clock(img):
-    """``torch_rot90_3D(img, ax=0, mode='clock')`` (torch_image_util.py:441-458) on (..., B, H, W):
+    """``torch_rot90_3D(img, ax=0, mode='clock')`` (torch_image_util.py:117-147, :130-132) on (..., B, H, W):
     transpose(1,2) then flip the last axis -> out[b,i,j] = img[b, H-1-j, i]."""
     return np.flip(np.swapaxes(img, -1, -2), axis=-1)
 
@@ -278,7 +278,7 @@ def xrf_atten_from_mask(mu_line, mask):
 
 
 def rotate_torch(x, theta, dtype="float32"):
-    """``torch_rotate_image_nd(x, theta)`` (torch_image_util.py:337-438) with its defaults
+    """``torch_rotate_image_nd(x, theta)`` (torch_image_util.py:13-114) with its defaults
     (enable_crop=True, align_corners=False, bilinear, zeros), executed with the container's
     torch on CPU.  x: (..., H, W) numpy; theta in radians (float32 tensor in the reference's
     callers, core.py:35,67)."""
@@ -290,7 +290,7 @@ def rotate_torch(x, theta, dtype="float32"):
     H, W = shape[-2:]
     xt = xt.reshape(-1, 1, H, W)
     N = xt.shape[0]
-    th = float(np.float32(theta))  # math.cos(theta) of a float32 tensor, as at :401-402
+    th = float(np.float32(theta))  # math.cos(theta) of a float32 tensor, as at :77-78
     cos_t, sin_t = math.cos(th), math.sin(th)
     mat = torch.tensor([[cos_t, -sin_t, 0.0], [sin_t, cos_t, 0.0]], dtype=tdt).unsqueeze(0).repeat(N, 1, 1)
     grid = F.affine_grid(mat, size=(N, 1, H, W), align_corners=False)
@@ -409,15 +409,67 @@ def ray_splat_mask(pz, py, q, wgt, L):
     return M
 
 
-def cone_march_f64(mu, pz, py, q, wgt):
-    """Direct FP64 march of the direction-sampled cone on mu (K, Z, X, Y), cone along +X."""
+def cone_march_f64(mu, pz, py, q, wgt, z0=0, z1=None, y0=0, y1=None):
+    """Direct FP64 march of the direction-sampled cone on mu (K, Z, X, Y), cone along +X.
+    With z0/z1 (y0/y1), only outputs z0 <= z < z1 (y0 <= y < y1); all of mu is read:
+    (K, z1-z0, X, y1-y0)."""
     mu = _f64(mu)
     K, Z, X, Y = mu.shape
+    z1 = Z if z1 is None else int(z1)
+    y1 = Y if y1 is None else int(y1)
+    if not (0 <= z0 < z1 <= Z and 0 <= y0 < y1 <= Y):
+        raise ValueError("need 0 <= z0 < z1 <= Z and 0 <= y0 < y1 <= Y")
     pz, py, q = (np.ascontiguousarray(v, dtype=np.int32) for v in (pz, py, q))
     wgt = _f64(wgt)
-    out = np.empty_like(mu)
+    out = np.empty((K, z1 - z0, X, y1 - y0))
     for k in range(K):
-        lib().or_cone_march_f64(_p(mu[k]), _p(out[k]), _i(Z), _i(X), _i(Y), _p(pz), _p(py), _p(q), _p(wgt), _i(len(pz)))
+        lib().or_cone_march_box_f64(_p(mu[k]), _p(out[k]), _i(Z), _i(X), _i(Y), _i(z0), _i(z1), _i(y0), _i(y1),
+                                    _p(pz), _p(py), _p(q), _p(wgt), _i(len(pz)))
+    return out
+
+
+def total_atten_cone_f64(frac, angle_deg, coef_x, coef_xrf, cone, b0=0, b1=None, r0=0, r1=None):
+    """Total attenuation A[e] = exp(-Cone(mu_e)) * incident_e at one angle, detector 'r',
+    for output slices b0 <= b < b1 and rows r0 <= r < r1: (K, b1-b0, r1-r0, W).  The
+    reference's driver chain (FL/core.py:19-101: rotate, torch_extract_3D_mu_xray / _xrf,
+    torch_xray_atten_3D, torch_xrf_atten_3D with the cone's splat mask, product) with the
+    rotation run by the container's torch in float32 (the reference's own op, rotate_torch)
+    and every later step in float64 (incident_f64, the direct cone march).  In the mask
+    frame of detector 'r' row r is y = H-1-r."""
+    frac = _f32(frac)
+    K, B, H, W = frac.shape
+    b1 = B if b1 is None else int(b1)
+    r1 = H if r1 is None else int(r1)
+    rot = rotate_torch(frac, np.float32(angle_deg / 180.0 * np.pi))
+    ax = incident_f64(mu_xray(rot[:, b0:b1], coef_x))[:, :, r0:r1]
+    mu_line = mu_xrf(rot, coef_xrf).astype(np.float64)
+    T = cone_march_f64(np.ascontiguousarray(rot90_clock(mu_line)), cone.pz, cone.py, cone.qd,
+                       np.asarray(cone.weight, np.float64), b0, b1, H - r1, H - r0)
+    # T[k, b, x=col, y'] with y' = (H-1-r) - (H-r1) = r1-1-r: row r is y' reversed
+    T = np.swapaxes(T, -1, -2)[:, :, ::-1, :]
+    return np.exp(-T) * ax
+
+
+def model_forward_f64(C, angles_deg, coef_x, coef_xrf, em, cone):
+    """Pf (K, A, B, W) of the attenuated model in float64 (detector 'r'): per angle the
+    total attenuation of C (total_atten_cone_f64, rotation in float64 by the container's
+    torch) and the projector oracle forward_batch_f64 per line with em[a, e]."""
+    C = _f64(C)
+    K, B, H, W = C.shape
+    A = len(angles_deg)
+    out = np.empty((K, A, B, W))
+    for a, ang in enumerate(angles_deg):
+        th = np.float32(ang / 180.0 * np.pi)
+        rot = rotate_torch(C, th, dtype="float64")
+        ax = incident_f64(rot * np.asarray(coef_x[a], np.float32).astype(np.float64)[:, None, None, None])
+        cf = np.asarray(coef_xrf[a], np.float32).astype(np.float64)
+        mu_line = np.einsum("jbhw,je->ebhw", rot, cf)
+        T = cone_march_f64(np.ascontiguousarray(rot90_clock(mu_line)), cone.pz, cone.py, cone.qd,
+                           np.asarray(cone.weight, np.float64))
+        At = np.exp(-np.ascontiguousarray(rot90_cclock(T))) * ax
+        for e in range(K):
+            out[e, a] = forward_batch_f64(At[e][None], C[e][None], np.array([[em[a, e]]], np.float64),
+                                          np.array([th], np.float64))[0]
     return out
 
 

@@ -1,9 +1,10 @@
 """Pure-torch restatement of the attenuated XRF forward model -- TEST INFRASTRUCTURE ONLY.
 
 Differentiable end to end with torch autograd (run in float64 on the CPU), so
-its gradient is the reference for the hand-written adjoint.  Each step is the
+its gradient is the reference for the hand-written adjoint.  Pinned by FP64
+central differences and by the oracle's kernels (tests/test_torch_model.py).  Each step is the
 reference's own tensor code path:
-  rotation     F.affine_grid + F.grid_sample            FL/torch_image_util.py:401-428
+  rotation     F.affine_grid + F.grid_sample            FL/torch_image_util.py:77-104
   incident     1 - mu_x, flip, cumprod, flip           FL/torch_functions.py:92-102
   fluorescence rot90 'clock', mask correlation, rot90 back, exp(-.)
                                                        FL/torch_functions.py:140-174 (atten_kernel as conv3d)
@@ -41,11 +42,21 @@ def incident(mux):
 
 
 def mask_correlation(mu, mask):
-    """atten_kernel on mu (K, Z, X, Y) as a conv3d (cross-correlation) with zero padding."""
+    """atten_kernel on mu (K, Z, X, Y) as a conv3d (cross-correlation) with zero padding.
+    A mask with few non-zero taps (a cone's splat mask) is applied tap by tap instead:
+    the same sum, as shifted multiply-adds of the padded volume."""
     Mz, Mx, My = mask.shape
     hz, hy = Mz // 2, My // 2
     t = F.pad(mu[:, None], (hy, My - 1 - hy, 0, Mx - 1, hz, Mz - 1 - hz))
-    return F.conv3d(t, mask[None, None])[:, 0]
+    nz = torch.nonzero(mask)
+    if len(nz) > 0.25 * mask.numel():
+        return F.conv3d(t, mask[None, None])[:, 0]
+    K, Z, X, Y = mu.shape
+    t = t[:, 0]
+    out = torch.zeros_like(mu)
+    for dz, dx, dy in nz.tolist():
+        out = out + mask[dz, dx, dy] * t[:, dz:dz + Z, dx:dx + X, dy:dy + Y]
+    return out
 
 
 def xrf(mu, mask, side="r"):

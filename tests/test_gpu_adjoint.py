@@ -160,3 +160,25 @@ def test_attenuation_streams_agree(kind):
     for n in (2, 3):
         gn = model.vjp(Cd, G, angle_chunk=2, streams=n).cpu().numpy()
         assert rel_l2(gn, g1) <= 1e-6, n
+
+
+def test_stored_exponents_over_streams():
+    """forward with stored exponents over 2 streams, then vjp over 2 streams, against the
+    one-stream forward and vjp; and the autograd function with streams=1."""
+    from fl_correction_amd.model import XRFModel, xrf_forward_autograd
+    C, param, angles, det, mask, em, coef_x, coef_xrf = _model_case("cone", seed=23)
+    one = XRFModel(param, angles, det, em=em, store=True)
+    two = XRFModel(param, angles, det, em=em, store=True)
+    Cd = _dev(C)
+    P1 = one.forward(Cd, angle_chunk=2, streams=1)
+    P2 = two.forward(Cd, angle_chunk=2, streams=2)
+    assert two._T_store is not None and torch.equal(P1, P2)
+    G = torch.randn_like(P1)
+    g1 = one.vjp(Cd, G, angle_chunk=2, streams=1)
+    g2 = two.vjp(Cd, G, angle_chunk=2, streams=2)
+    assert rel_l2(g2.cpu().numpy(), g1.cpu().numpy()) <= 1e-6
+    Cg = Cd.clone().requires_grad_(True)
+    Pf = xrf_forward_autograd(Cg, one, streams=1)
+    assert torch.equal(Pf.detach(), P1)
+    (Pf * G).sum().backward()
+    assert torch.equal(Cg.grad, one.vjp(Cd, G, streams=1))

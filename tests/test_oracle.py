@@ -15,6 +15,9 @@ function is checked against something that fixes its answer independently:
     against the closed-form FP64 statement;
   * cone: the ray-splat mask correlated by the mask kernel against the direct march.
 """
+import os
+import sys
+
 import numpy as np
 import pytest
 import torch
@@ -370,3 +373,69 @@ def test_legacy_mlem_raises_the_poisson_likelihood():
         pf = np.maximum(Hm @ est, 1e-12)
         ll.append(float(np.sum(y * np.log(pf) - pf)))
     assert all(b >= a - 1e-9 * abs(a) for a, b in zip(ll, ll[1:]))
+
+
+# ---------------------------------------------------------------------------
+# FP64 chain of the cone attenuation and of the whole model (used at the
+# workloads' own sizes), and the crop case's central-difference fixture
+# ---------------------------------------------------------------------------
+def test_cone_chain_f64_equals_mask_chain_and_boxes():
+    from fl_correction_amd.detector_mask import Cone
+    rng = np.random.default_rng(31)
+    frac = rng.uniform(0, 1, (2, 6, 20, 24)).astype(np.float32)
+    cone = Cone(n=3, half_angle_deg=15, q=8)
+    cx = np.array([0.02, 0.03], np.float32)
+    cf = rng.uniform(0.01, 0.03, (2, 2)).astype(np.float32)
+    for ang in (0.0, 33.0, 121.0):
+        full = oracle.total_atten_cone_f64(frac, ang, cx, cf, cone)
+        part = oracle.total_atten_cone_f64(frac, ang, cx, cf, cone, 2, 4, 5, 13)
+        assert np.array_equal(full[:, 2:4, 5:13], part)
+        rot = oracle.rotate_torch(frac, np.float32(ang / 180 * np.pi))
+        ref = oracle.xrf_atten_from_mask(oracle.mu_xrf(rot, cf), cone.splat_mask(24)) * \
+            oracle.incident_f32(oracle.mu_xray(rot, cx))
+        assert rel_l2(ref, full) < 1e-6
+
+
+def test_model_forward_f64_equals_torch_model():
+    import torch_model as tm
+    from fl_correction_amd.detector_mask import Cone
+    rng = np.random.default_rng(32)
+    K, B, N, A = 2, 4, 14, 3
+    C = rng.uniform(0, 1, (K, B, N, N))
+    angles = np.array([0.0, 40.0, 115.0])
+    cx = rng.uniform(0.01, 0.04, (A, K)).astype(np.float32)
+    cf = rng.uniform(0.005, 0.03, (A, K, K)).astype(np.float32)
+    em = rng.uniform(0.5, 1.5, (A, K)).astype(np.float32)
+    cone = Cone(n=3, half_angle_deg=15, q=8)
+    a = oracle.model_forward_f64(C, angles, cx, cf, em, cone)
+    b = tm.forward(torch.tensor(C), angles, cx, cf, em, cone.splat_mask(N)).numpy()
+    # the torch model rounds the rotation matrix to float32 as the reference's float32 callers do
+    assert rel_l2(a, b) < 1e-6
+
+
+def test_torch_model_sparse_taps_equal_conv3d():
+    import torch_model as tm
+    from fl_correction_amd.detector_mask import Cone
+    rng = np.random.default_rng(33)
+    mu = torch.tensor(rng.uniform(0, 1, (2, 7, 16, 9)))
+    M = torch.tensor(Cone(n=3, half_angle_deg=20, q=8).splat_mask(16).astype(np.float64))
+    assert (M != 0).sum() < 0.25 * M.numel()
+    sparse = tm.mask_correlation(mu, M)
+    import torch.nn.functional as F
+    Mz, Mx, My = M.shape
+    t = F.pad(mu[:, None], (My // 2, My - 1 - My // 2, 0, Mx - 1, Mz // 2, Mz - 1 - Mz // 2))
+    dense = F.conv3d(t, M[None, None])[:, 0]
+    assert torch.allclose(sparse, dense, rtol=1e-12, atol=1e-14)
+
+
+def test_crop_fixture_central_differences():
+    """Two of the fixture's 64 FP64 central differences, recomputed from the oracle."""
+    import helpers
+    sys.path.insert(0, os.path.join(os.path.dirname(__file__), "golden"))
+    import make_model_crop_fd as mk
+    fx = np.load(os.path.join(os.path.dirname(__file__), "golden", "model_crop_fd.npz"))
+    case = helpers.model_crop_case()
+    assert np.array_equal(fx["idx"], mk.voxel_indices(case["C_eval"].shape))
+    for k in (0, 37):
+        fd = mk.central_difference(case, fx["idx"][k])
+        assert abs(fd - fx["fd"][k]) <= 1e-9 * np.abs(fx["fd"]).max()
