@@ -14,7 +14,8 @@
   same results bit for bit, the vjp agrees with a central difference along a random
   direction, attenuation and projections on a slab against the oracle.
 * batch_48: the 48 uneven samples, 20 iterations under the hipGraph, one slice of every
-  problem against the FP32 oracle.
+  problem against the FP32 oracle (1e-5), or, for the few problems where the FP32 oracle's
+  own serial order is already ~1e-5 from the FP64 oracle, no farther than 2x that from FP64.
 * joint_512 (128 x 512^2, 5 lines, 7x7 cone over 20 deg): two angles at full size, the
   attenuation on a strip of one slice and the projections of that slice against the oracle,
   the vjp against a central difference along a random direction.
@@ -297,14 +298,25 @@ def test_batch_48_samples_against_oracle():
     run = BatchMLEM(probs, beta=0.0)
     run.run(20, graph=True)
     torch.cuda.synchronize()
-    worst = 0.0
+    worst, floor = 0.0, []
     for i, (p, (a, em, th, I, b)) in enumerate(zip(probs, host)):
         got = p.C[b].cpu().numpy()
-        ref = oracle.mlem_batch_f32(np.ones((1, 1) + got.shape, np.float32), a, em[:, None], th, I, 20, 0.0, 0.01)
-        err = rel_l2(got, ref[0, 0])
+        C0 = np.ones((1, 1) + got.shape, np.float32)
+        ref = oracle.mlem_batch_f32(C0, a, em[:, None], th, I, 20, 0.0, 0.01)[0, 0]
+        err = rel_l2(got, ref)
         worst = max(worst, err)
-        assert err <= RTOL, (i, err)
-    _say(f"  one slice of each of {len(probs)} problems after 20 iterations: worst rel-L2 vs FP32 oracle {worst:.2e}")
+        if err > RTOL:
+            # 20 iterations on noisy data, mostly zero background: here two FP32 summation
+            # orders can differ by ~1e-5.  Then the result must be as close to the FP64
+            # oracle as the FP32 oracle's own serial order is (within 2x).
+            ref64 = oracle.mlem_batch_f64(C0, a, em[:, None], th, I, 20, 0.0, 0.01)[0, 0]
+            e_hip, e_ref = rel_l2(got, ref64), rel_l2(ref, ref64)
+            floor.append((i, err, e_hip, e_ref))
+            _say(f"  problem {i}: rel-L2 vs FP32 oracle {err:.2e}; vs FP64: this {e_hip:.2e}, FP32 oracle {e_ref:.2e}")
+            assert e_ref >= 0.5 * RTOL and e_hip <= 2 * e_ref, (i, err, e_hip, e_ref)
+    _say(f"  one slice of each of {len(probs)} problems after 20 iterations: worst rel-L2 vs FP32 oracle {worst:.2e}; "
+         f"{len(floor)} above 1e-5, each within 2x of the FP32 oracle's own distance to FP64")
+    assert len(floor) <= len(probs) // 20
     run.close()
 
 

@@ -163,3 +163,20 @@ def test_errors(ops):
                                    torch.zeros(2, device="cuda"))  # batch mismatch
     with pytest.raises(RuntimeError):
         ops.forward_emission_batch(a[0], a, torch.ones(2, 1, device="cuda"), torch.zeros(2, device="cuda"))
+
+
+@pytest.mark.parametrize("shape,delta", [((2, 3, 40, 70), 0.01), ((1, 1, 5, 2), 0.05), ((3, 2, 64, 64), 1e-3)])
+def test_huber_grad_batch_against_oracle(shape, delta):
+    """fl_huber_grad_batch (the reference's internal huber_grad_batch, atten_cuda.cpp:229-238)
+    against the FP32 oracle: the same four clipped differences summed in the same order."""
+    import torch
+    from fl_correction_amd import _lib
+    rng = np.random.default_rng(61)
+    C = rng.uniform(0, 1, shape).astype(np.float32)
+    C[..., ::3, ::2] = C[..., ::3, ::2] * 0.001  # steps both below and above delta
+    Cd = torch.from_numpy(C).cuda()
+    g = torch.full_like(Cd, 99.0)
+    R, B, H, W = shape
+    _lib.call("fl_huber_grad_batch", _lib.ptr(Cd), _lib.ptr(g), R, B, H, W, float(delta), _lib.stream_of(Cd.device))
+    ref = oracle.huber_batch_f32(C, delta)
+    assert np.array_equal(g.cpu().numpy(), ref)
