@@ -37,6 +37,7 @@ SIGNATURES = {
     "fl_huber_grad_batch": (_INT, [_P, _P, _I64, _I64, _I64, _I64, _F, _P]),
     "fl_mlem_batch": (_INT, [_P, _P, _P, _P, _P, _P, _I64, _I64, _I64, _I64, _I64, _I64, _D, _D, _P, _SZ, _P]),
     "fl_mlem_workspace_bytes": (_SZ, [_I64, _I64, _I64, _I64, _I64]),
+    "fl_mlem_batch_noalloc": (_INT, [_P, _P, _P, _P, _P, _P, _I64, _I64, _I64, _I64, _I64, _I64, _D, _D, _P]),
     "fl_mlem_sensitivity": (_INT, [_P, _P, _P, _P, _I64, _I64, _I64, _I64, _I64, _P]),
     "fl_mlem_ratio": (_INT, [_P, _P, _P, _P, _P, _P, _I64, _I64, _I64, _I64, _I64, _P]),
     "fl_mlem_update": (_INT, [_P, _P, _P, _P, _P, _P, _P, _I64, _I64, _I64, _I64, _I64, _D, _D, _P]),

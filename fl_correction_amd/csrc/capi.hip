@@ -1,9 +1,10 @@
 // C ABI entry points (declared in include/fl_correction_amd.h).  Each checks
 // its arguments on the host, then enqueues kernels on the caller's stream.
 // No entry point allocates device memory: scratch comes from the caller's
-// workspace argument.
+// workspace argument, or for fl_mlem_batch_noalloc from a static device array.
 #include "common.hpp"
 
+#include <mutex>
 #include <string>
 
 namespace fl {
@@ -49,6 +50,37 @@ static int check_dims(int64_t A, int64_t B, int64_t R, int64_t H, int64_t W)
                "shape: dimension too large");
     // one slice is addressed with 32-bit byte offsets
     FL_REQUIRE(H * W < (int64_t)1 << 29, "shape: H*W must be below 2^29");
+    return FL_OK;
+}
+
+// ---------------------------------------------------------------------------
+// fl_mlem_batch without a workspace: its scratch is a file-scope device array.
+// Every GPU the library runs on holds its own copy of the array (a module
+// global) until the process ends.  In host code `scratch` is only a handle:
+// its device address on the current device comes from hipGetSymbolAddress,
+// looked up once per device and kept in g_scratch_addr.
+// ---------------------------------------------------------------------------
+
+static __device__ __align__(256) unsigned char scratch[FL_MLEM_NOALLOC_SCRATCH_BYTES];
+
+constexpr int kMaxDevices = 64;
+static std::mutex g_scratch_mu;
+static void *g_scratch_addr[kMaxDevices] = {};
+
+// Device address of this device's `scratch` (the current device must be `dev`).
+static int scratch_address(int dev, void **out)
+{
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch_addr[dev]) {
+        void *p = nullptr;
+        hipError_t err = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (err != hipSuccess || !p) {
+            set_error(std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(err));
+            return FL_ERR_HIP;
+        }
+        g_scratch_addr[dev] = p;
+    }
+    *out = g_scratch_addr[dev];
     return FL_OK;
 }
 
@@ -202,6 +234,38 @@ int fl_mlem_batch(const float *C_init, const float *atten, const float *em_cs,
     if (cur != C_out)
         FL_HIP_CHECK(hipMemcpyAsync(C_out, cur, nC * sizeof(float), hipMemcpyDeviceToDevice, s));
     return FL_OK;
+}
+
+int fl_mlem_batch_noalloc(const float *C_init, const float *atten, const float *em_cs,
+                          const float *theta, const float *I, float *C_out, int64_t A, int64_t B,
+                          int64_t R, int64_t H, int64_t W, int64_t n_iter, double beta, double delta,
+                          void *stream)
+{
+    g_last_error.clear();
+    if (int e = check_dims(A, B, R, H, W)) return e;
+    const size_t need = fl_mlem_workspace_bytes(A, B, R, H, W);
+    if (need > (size_t)FL_MLEM_NOALLOC_SCRATCH_BYTES) {
+        set_error("fl_mlem_batch_noalloc: the problem needs " + std::to_string(need) +
+                  " bytes of scratch, the static scratch holds " +
+                  std::to_string((size_t)FL_MLEM_NOALLOC_SCRATCH_BYTES) +
+                  "; call fl_mlem_batch with a workspace instead");
+        return FL_ERR_WORKSPACE;
+    }
+    int dev = -1;
+    FL_HIP_CHECK(hipGetDevice(&dev));
+    FL_REQUIRE(dev >= 0 && dev < kMaxDevices, "fl_mlem_batch_noalloc: device ordinal out of range");
+    if (stream) {
+        int sdev = -1;
+        FL_HIP_CHECK(hipStreamGetDevice((hipStream_t)stream, &sdev));
+        FL_REQUIRE(sdev == dev, "fl_mlem_batch_noalloc: the stream belongs to another device than the current one");
+    }
+    void *ws = nullptr;
+    if (int e = scratch_address(dev, &ws)) return e;
+    // every region of the scratch is written before it is read (sens by the sensitivity
+    // pass, the ratio by each forward pass, the second estimate by each update), so the
+    // previous call's contents need no clearing
+    return fl_mlem_batch(C_init, atten, em_cs, theta, I, C_out, A, B, R, H, W, n_iter, beta, delta, ws,
+                         (size_t)FL_MLEM_NOALLOC_SCRATCH_BYTES, stream);
 }
 
 // ---------------------------------------------------------------------------

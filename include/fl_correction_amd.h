@@ -97,6 +97,21 @@ FL_API int fl_mlem_batch(const float *C_init, const float *atten, const float *e
                   void *workspace, size_t workspace_bytes, void *stream);
 FL_API size_t fl_mlem_workspace_bytes(int64_t A, int64_t B, int64_t R, int64_t H, int64_t W);
 
+/* fl_mlem_batch with no workspace argument and no allocation of any kind: the scratch is
+ * a static device array of FL_MLEM_NOALLOC_SCRATCH_BYTES in the library (each GPU the
+ * library runs on holds one until the process ends).  No counterpart in the reference:
+ * for callers that cannot allocate device memory (code under HIP graph capture, tight
+ * serving loops, harnesses that forbid hipMalloc).  Problems whose
+ * fl_mlem_workspace_bytes exceeds the array fail with FL_ERR_WORKSPACE.  All calls on one
+ * GPU share that GPU's array: calls that overlap on one GPU (other streams, other host
+ * threads) race and must be serialised by the caller.  The stream must belong to the
+ * current device. */
+#define FL_MLEM_NOALLOC_SCRATCH_BYTES (64u << 20)
+FL_API int fl_mlem_batch_noalloc(const float *C_init, const float *atten, const float *em_cs,
+                                 const float *theta, const float *I, float *C_out, int64_t A,
+                                 int64_t B, int64_t R, int64_t H, int64_t W, int64_t n_iter,
+                                 double beta, double delta, void *stream);
+
 /* The three stages of fl_mlem_batch, for callers that drive the iteration
  * themselves (and time its kernels):
  *   fl_mlem_sensitivity: sens = max(B^T 1, 1e-6)                  (atten_cuda.cpp:308-314)
