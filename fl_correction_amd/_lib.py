@@ -38,6 +38,13 @@ SIGNATURES = {
     "fl_mlem_batch": (_INT, [_P, _P, _P, _P, _P, _P, _I64, _I64, _I64, _I64, _I64, _I64, _D, _D, _P, _SZ, _P]),
     "fl_mlem_workspace_bytes": (_SZ, [_I64, _I64, _I64, _I64, _I64]),
     "fl_mlem_batch_noalloc": (_INT, [_P, _P, _P, _P, _P, _P, _I64, _I64, _I64, _I64, _I64, _I64, _D, _D, _P]),
+    "fl_slice_major": (_INT, [_P, _P, _I64, _I64, _I64, _P]),
+    "fl_backproject_sm_scratch_bytes": (_SZ, [_I64, _I64, _I64, _I64]),
+    "fl_backward_emission_sm": (_INT, [_P, _P, _P, _P, _P, _I64, _I64, _I64, _I64, _INT, _P, _SZ, _P]),
+    "fl_mlem_ratio_sm": (_INT, [_P, _P, _P, _P, _P, _P, _I64, _I64, _I64, _I64, _P]),
+    "fl_mlem_sensitivity_sm": (_INT, [_P, _P, _P, _P, _I64, _I64, _I64, _I64, _P, _SZ, _P]),
+    "fl_mlem_update_sm": (_INT, [_P, _P, _P, _P, _P, _P, _P, _I64, _I64, _I64, _I64, _D, _D, _P, _SZ, _P]),
+    "fl_mlem_workspace_bytes_sm": (_SZ, [_I64, _I64, _I64, _I64, _I64]),
     "fl_mlem_sensitivity": (_INT, [_P, _P, _P, _P, _I64, _I64, _I64, _I64, _I64, _P]),
     "fl_mlem_ratio": (_INT, [_P, _P, _P, _P, _P, _P, _I64, _I64, _I64, _I64, _I64, _P]),
     "fl_mlem_update": (_INT, [_P, _P, _P, _P, _P, _P, _P, _I64, _I64, _I64, _I64, _I64, _D, _D, _P]),

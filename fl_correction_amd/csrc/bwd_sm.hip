@@ -1,0 +1,787 @@
+// Back-projection (exact transpose of the forward projector's sampling) on a
+// slice-major copy of the attenuation, for CDNA4.
+//
+// In the reference's layout atten[a,b,h,w] a back-projector that owns an image
+// tile reads short row segments of every detector row it needs, and each
+// sample's position and bilinear weights are computed once per slice.  The
+// slice-major copy atten_sm[a,h,w,b] puts the B slices of one detector point in
+// contiguous memory.  A wavefront here covers 64 * SPL slices (lane = SPL
+// consecutive slices, one 4*SPL-byte load per point): the position, the clamping
+// and the four weights of a detector point are the same for all of them and are
+// computed once, and each lane adds into its own slices' accumulators, so no two
+// lanes ever touch the same word.
+//
+// The detector points are partitioned by where the forward projector's sample
+// of each lands, and every point is taken by exactly one work item:
+//  * Interior blocks (sm_interior_kernel): points whose sample lies inside the
+//    clamp box [0, H-1.001] x [0, W-1.001] and whose base cell (floor of the
+//    position) is in one 4 x 4 block of base cells.  Lane 4c + q tests candidate
+//    q of cell c (the 2 x 2 lattice points of the bounding box of the cell's
+//    preimage, which hold every point of the cell); a quad permute then packs
+//    each cell's hits into its first lanes.  The block's 5 x 5 window of voxels
+//    (cells plus their lower/right neighbours) is 25 * SPL registers per lane,
+//    and every hit's four targets are registers named at compile time by its
+//    cell.  Hits are taken in passes (pass s = the s-th hit of every cell,
+//    usually two passes); a pass walks the 16 cells in a fixed unrolled order
+//    with no branches, loads issued a ring of cells ahead of their multiply-adds
+//    (a cell without a hit in the pass reads through an out-of-range offset,
+//    which returns 0).  The MLEM ratio rows the block needs at an angle are
+//    cached in LDS.  The 4 x 4 part of the window is the block's own output; its
+//    last row and column (the halo) go to a small buffer.
+//  * Edge strips (sm_edge_kernel): points whose sample the reference clamps onto
+//    the image border (.cu:419-420), split by the border they clamp onto
+//    (row < 0: top, row > H-1.001: bottom, else col < 0: left, col > W-1.001:
+//    right), by an 8-wide strip of border base cells, by 8-voxel depth bins and
+//    by ranges of angles, so that no item holds a long chain of work.  A strip
+//    enumerates the lattice points of the bounding box of its region's preimage,
+//    64 per round, and read-add-writes its 2 x 9 (or 9 x 2) window of border
+//    voxels in LDS with the clamped position's weights.
+// sm_finalize adds, per voxel and in a fixed order, the owning block's sum, the
+// halos of the blocks above and to the left, and every edge strip covering the
+// voxel, transposes back to the reference's (B, H, W) layout through LDS and
+// applies the output mode (write, sensitivity, MLEM/OSL update, accumulate).
+// No atomics anywhere: results are reproducible bit for bit.
+#include "projector.hpp"
+
+#include <algorithm>
+#include <cmath>
+
+namespace fl {
+
+constexpr int kSmT = 8;                  // edge strips: 8 border base cells, 8-voxel depth bins
+constexpr int kSmWin = kSmT + 1;         // edge window length along the border
+constexpr int kSmEdgeVox = 2 * kSmWin;   // voxels of an edge strip's window
+constexpr int kSmPre = 8;                // edge hits whose loads are in flight together
+constexpr int kSmTB = 4;                 // interior block: 4 x 4 base cells
+constexpr int kSmWB = kSmTB + 1;         // interior window side
+constexpr int kSmWBVox = kSmWB * kSmWB;  // 25
+constexpr int kSmHaloB = 2 * kSmTB + 1;  // halo of a block: row of 5, column of 4
+constexpr int kSmRatRows = 12;           // ratio rows cached per interior block and angle
+constexpr uint32_t kSmOff = 0x7FFFFFF0u; // an offset past every buffer (reads 0)
+
+enum SmKind { kSmTop = 0, kSmBottom = 1, kSmLeft = 2, kSmRight = 3 };
+
+struct SmGeom {
+    int A, B, H, W;
+    // edge strips
+    int nsi, nsj;    // strips along the left/right borders (rows) and top/bottom (columns)
+    int nd;          // depth bins
+    int nch;         // angle ranges per strip
+    float reach;     // bound on |position - centre| over the detector, plus a margin
+    // 64 * spl slices per wavefront, in Gi groups
+    int spl, S, Gi;
+    int nbi, nbj;    // 4 x 4 blocks over base cells 0..H-2 x 0..W-2
+};
+
+struct SmArgs {
+    SmGeom g;
+    const float *atten;  // (A, H, W, B)
+    const float *ratio;  // (A, W, B), or null for ratio = 1
+    const float *em;     // (A): one reference per line
+    const float *theta;
+    float *own;          // (Gi, nbi*4, nbj*4, S)
+    float *halo;         // (Gi, nbi, nbj, kSmHaloB, S)
+    float *edge;         // (Gi, 4, max(nsi, nsj), nd, nch, kSmEdgeVox, S)
+    float *edge_sum;     // (Gi, 4, max(nsi, nsj), kSmEdgeVox, S)
+};
+
+__host__ __device__ inline int sm_edge_items(const SmGeom &g) { return (2 * g.nsj + 2 * g.nsi) * g.nd * g.nch; }
+
+__host__ __device__ inline int64_t sm_edge_index(const SmGeom &g, int grp, int side, int t, int d, int ch)
+{
+    const int ns = g.nsi > g.nsj ? g.nsi : g.nsj;
+    return ((((int64_t)grp * 4 + side) * ns + t) * g.nd + d) * g.nch + ch;
+}
+
+// Region of an edge strip in sample coordinates (row, col) and the origin of its
+// voxel window.
+struct SmItem {
+    int side, t, d, ch;
+    float r0, r1, c0, c1;
+    int wy, wx;
+};
+
+__device__ __forceinline__ SmItem sm_item(const SmGeom &g, int k, float xmax, float ymax)
+{
+    SmItem it;
+    const float cx = (g.W - 1) * 0.5f, cy = (g.H - 1) * 0.5f;
+    const float rlo = cy - g.reach, rhi = cy + g.reach, clo = cx - g.reach, chi = cx + g.reach;
+    it.ch = k % g.nch;
+    k /= g.nch;
+    const int ntb = g.nsj * g.nd;
+    if (k < 2 * ntb) {
+        const bool top = k < ntb;
+        if (!top) k -= ntb;
+        it.side = top ? kSmTop : kSmBottom;
+        it.t = k / g.nd;
+        it.d = k % g.nd;
+        const bool last_d = it.d == g.nd - 1;
+        if (top) {
+            it.r0 = last_d ? rlo : -(float)(8 * (it.d + 1));
+            it.r1 = -(float)(8 * it.d);
+            it.wy = 0;
+        } else {
+            it.r0 = xmax + (float)(8 * it.d);
+            it.r1 = last_d ? rhi : xmax + (float)(8 * (it.d + 1));
+            it.wy = g.H - 2;
+        }
+        it.c0 = it.t == 0 ? clo : (float)(8 * it.t);
+        it.c1 = it.t == g.nsj - 1 ? chi : (float)(8 * it.t + 8);
+        it.wx = 8 * it.t;
+        return it;
+    }
+    k -= 2 * ntb;
+    const int nlr = g.nsi * g.nd;
+    const bool left = k < nlr;
+    if (!left) k -= nlr;
+    it.side = left ? kSmLeft : kSmRight;
+    it.t = k / g.nd;
+    it.d = k % g.nd;
+    const bool last_d = it.d == g.nd - 1;
+    it.r0 = (float)(8 * it.t);
+    it.r1 = (float)(8 * it.t + 8);
+    if (left) {
+        it.c0 = last_d ? clo : -(float)(8 * (it.d + 1));
+        it.c1 = -(float)(8 * it.d);
+        it.wx = 0;
+    } else {
+        it.c0 = ymax + (float)(8 * it.d);
+        it.c1 = last_d ? chi : ymax + (float)(8 * (it.d + 1));
+        it.wx = g.W - 2;
+    }
+    it.wy = 8 * it.t;
+    return it;
+}
+
+// Depth bin of a distance v >= 0 beyond the clamp box.
+__device__ __forceinline__ int sm_depth(float v, int nd) { return min((int)(v * 0.125f), nd - 1); }
+
+template <int N>
+struct alignas(4 * N) SmVec {
+    float v[N];
+};
+
+template <int N, int AUX>
+__device__ __forceinline__ SmVec<N> sm_load(__amdgpu_buffer_rsrc_t r, uint32_t off)
+{
+    SmVec<N> o;
+    if constexpr (N == 1) {
+        o.v[0] = __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(r, off, 0, AUX));
+    } else if constexpr (N == 2) {
+        const auto x = __builtin_amdgcn_raw_buffer_load_b64(r, off, 0, AUX);
+        o.v[0] = __uint_as_float(x[0]);
+        o.v[1] = __uint_as_float(x[1]);
+    } else {
+        const auto x = __builtin_amdgcn_raw_buffer_load_b128(r, off, 0, AUX);
+        o.v[0] = __uint_as_float(x[0]);
+        o.v[1] = __uint_as_float(x[1]);
+        o.v[2] = __uint_as_float(x[2]);
+        o.v[3] = __uint_as_float(x[3]);
+    }
+    return o;
+}
+
+// ---------------------------------------------------------------------------
+// Edge strips: the hits' base voxels vary per hit, so the window is an LDS array
+// (2 x 9 voxels for the top and bottom strips, 9 x 2 for the left and right ones,
+// pitch `pw`), 64 * SPL slices per voxel, read-add-written per hit.  The kSmEW
+// wavefronts of a workgroup share one item and take its angles in turn, each into
+// its own window; the windows are summed in wavefront order at the end.
+// ---------------------------------------------------------------------------
+constexpr int kSmEW = 4;  // wavefronts per edge item
+
+template <int SPL, bool RATIO>
+__global__ __launch_bounds__(64 * kSmEW) void sm_edge_kernel(SmArgs p)
+{
+    constexpr int S = 64 * SPL;
+    __shared__ SmVec<SPL> win_all[kSmEW * kSmEdgeVox * 64];
+    const SmGeom &g = p.g;
+    const int per_group = sm_edge_items(g);
+    const uint32_t id = xcd_remap(blockIdx.x, gridDim.x);
+    const int grp = (int)(id / (uint32_t)per_group);
+    const int k = (int)(id % (uint32_t)per_group);
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    SmVec<SPL> *win = win_all + wave * kSmEdgeVox * 64;
+    const int b0 = grp * S + lane * SPL;  // this lane's first slice (B is a multiple of SPL)
+    const float xmax = g.H - 1.001f, ymax = g.W - 1.001f;
+    const float cx = (g.W - 1) * 0.5f, cy = (g.H - 1) * 0.5f;
+    const SmItem it = sm_item(g, k, xmax, ymax);
+    const int64_t HWB = (int64_t)g.H * g.W * g.B;
+    const int64_t WB = (int64_t)g.W * g.B;
+    // lanes past the last slice read through an offset past every buffer (0)
+    const uint32_t voff = b0 < g.B ? (uint32_t)b0 * 4u : kSmOff;
+    const bool rows = it.side == kSmTop || it.side == kSmBottom;
+    const int pw = rows ? kSmWin : 2;  // window pitch (voxels)
+    const int a_begin = (int)((int64_t)g.A * it.ch / g.nch), a_end = (int)((int64_t)g.A * (it.ch + 1) / g.nch);
+
+    for (int v = 0; v < kSmEdgeVox; ++v)
+#pragma unroll
+        for (int j = 0; j < SPL; ++j) win[v * 64 + lane].v[j] = 0.0f;
+
+    // Every sample lies in the disc of radius `reach` about the centre (the
+    // rotation preserves distances): clip the region's columns to the disc's
+    // chord over its rows, and its rows to the chord over its columns.
+    float r0 = fminf(it.r0, it.r1), r1 = fmaxf(it.r0, it.r1), c0 = fminf(it.c0, it.c1), c1 = fmaxf(it.c0, it.c1);
+    {
+        const float R2 = g.reach * g.reach;
+        const float dr = fmaxf(0.0f, fmaxf(r0 - cy, cy - r1)), dc = fmaxf(0.0f, fmaxf(c0 - cx, cx - c1));
+        const float hc = sqrtf(fmaxf(R2 - dr * dr, 0.0f)), hr = sqrtf(fmaxf(R2 - dc * dc, 0.0f));
+        c0 = fmaxf(c0, cx - hc);
+        c1 = fminf(c1, cx + hc);
+        r0 = fmaxf(r0, cy - hr);
+        r1 = fminf(r1, cy + hr);
+    }
+    const bool reachable = r0 <= r1 && c0 <= c1;
+
+    for (int a = a_begin + wave; a < a_end && reachable; a += kSmEW) {
+        const float th = finite_or_zero(scalar_read(p.theta, a));
+        const float c = cosf(-th), s = sinf(-th);
+        // bounding box of the preimage of the item's region (inverse rotation of
+        // its corners), one lattice step of margin, clipped to the detector
+        float hlo = 3.0e38f, hhi = -3.0e38f, wlo = 3.0e38f, whi = -3.0e38f;
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            const float rr = ((q & 1) ? r1 : r0) - cy;
+            const float cc = ((q & 2) ? c1 : c0) - cx;
+            const float hh = c * rr + s * cc, ww = -s * rr + c * cc;
+            hlo = fminf(hlo, hh);
+            hhi = fmaxf(hhi, hh);
+            wlo = fminf(wlo, ww);
+            whi = fmaxf(whi, ww);
+        }
+        const int h0 = max(0, (int)floorf(hlo + cy) - 1), h1 = min(g.H - 1, (int)ceilf(hhi + cy) + 1);
+        const int w0 = max(0, (int)floorf(wlo + cx) - 1), w1 = min(g.W - 1, (int)ceilf(whi + cx) + 1);
+        if (h0 > h1 || w0 > w1) continue;
+        const int nw = w1 - w0 + 1;
+        const int ncand = (h1 - h0 + 1) * nw;
+        const float inv_nw = 1.0f / (float)nw;
+        const float em = scalar_read(p.em, a);
+        const __amdgpu_buffer_rsrc_t ra = make_rsrc(p.atten + (int64_t)a * HWB, (int)HWB);
+        __amdgpu_buffer_rsrc_t rq = ra;
+        if constexpr (RATIO) rq = make_rsrc(p.ratio + (int64_t)a * WB, (int)WB);
+
+        for (int base = 0; base < ncand; base += 64) {
+            // ---- candidate test: lane = one lattice point of the box ----------
+            const int kc = base + lane;
+            int qh = (int)(((float)kc + 0.5f) * inv_nw);
+            int qw = kc - qh * nw;
+            if (qw < 0) {
+                --qh;
+                qw += nw;
+            } else if (qw >= nw) {
+                ++qh;
+                qw -= nw;
+            }
+            const int h = h0 + qh, w = w0 + qw;
+            bool hit = kc < ncand;
+            // the forward projector's position arithmetic, unclamped
+            const float hc = (float)h - cy, wc = (float)w - cx;
+            const float xu = fmaf(c, hc, fmaf(-s, wc, cy));  // row
+            const float yu = fmaf(s, hc, fmaf(c, wc, cx));   // col
+            const bool rin = xu >= 0.0f && xu <= xmax;
+            const float xc = clampf(xu, 0.0f, xmax), yc = clampf(yu, 0.0f, ymax);
+            const int y0 = (int)xc, x0 = (int)yc;
+            const float fy = __builtin_amdgcn_fractf(xc), fx = __builtin_amdgcn_fractf(yc);
+            if (it.side == kSmTop) {
+                hit = hit && xu < 0.0f && (x0 >> 3) == it.t && sm_depth(-xu, g.nd) == it.d;
+            } else if (it.side == kSmBottom) {
+                hit = hit && xu > xmax && (x0 >> 3) == it.t && sm_depth(xu - xmax, g.nd) == it.d;
+            } else if (it.side == kSmLeft) {
+                hit = hit && rin && yu < 0.0f && (y0 >> 3) == it.t && sm_depth(-yu, g.nd) == it.d;
+            } else {
+                hit = hit && rin && yu > ymax && (y0 >> 3) == it.t && sm_depth(yu - ymax, g.nd) == it.d;
+            }
+            // per-hit fields: byte offsets of the point's slices and ratio, the
+            // window word of its base voxel, its four bilinear weights (lanes
+            // without a hit: an offset past the buffers and weights 0)
+            const uint32_t aoff = hit ? (uint32_t)((h * g.W + w) * g.B) * 4u : kSmOff;
+            const uint32_t roff = hit ? (uint32_t)(w * g.B) * 4u : kSmOff;
+            const int wv = hit ? (y0 - it.wy) * pw + (x0 - it.wx) : 0;
+            const float w00 = (1.0f - fy) * (1.0f - fx), w01 = (1.0f - fy) * fx;
+            const float w10 = fy * (1.0f - fx), w11 = fy * fx;
+            uint64_t mask = __ballot(hit);
+
+            // ---- walk the hits, kSmPre at a time.  The loads are unconditional
+            // (an empty slot reads 0 through an offset past the buffers), so the
+            // later ones stay in flight while the earlier hits are added
+            while (mask) {
+                int ln[kSmPre];
+                int nh = 0;
+#pragma unroll
+                for (int i = 0; i < kSmPre; ++i) {
+                    ln[i] = mask ? __builtin_ctzll(mask) : 64;
+                    nh += mask ? 1 : 0;
+                    mask &= mask - 1;
+                }
+                SmVec<SPL> val[kSmPre], rv[kSmPre];
+#pragma unroll
+                for (int i = 0; i < kSmPre; ++i) {
+                    const uint32_t ao = i < nh ? (uint32_t)__builtin_amdgcn_readlane((int)aoff, ln[i] & 63) : kSmOff;
+                    val[i] = sm_load<SPL, 2>(ra, voff + ao);
+                    if constexpr (RATIO) {
+                        const uint32_t ro = i < nh ? (uint32_t)__builtin_amdgcn_readlane((int)roff, ln[i] & 63) : kSmOff;
+                        rv[i] = sm_load<SPL, 0>(rq, voff + ro);
+                    }
+                }
+#pragma unroll
+                for (int i = 0; i < kSmPre; ++i) {
+                    if (i < nh) {  // wave-uniform
+                        const int l = ln[i];
+                        const int vv = __builtin_amdgcn_readlane(wv, l);
+                        const float a00 = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(w00), l));
+                        const float a01 = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(w01), l));
+                        const float a10 = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(w10), l));
+                        const float a11 = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(w11), l));
+                        SmVec<SPL> *o = win + vv * 64 + lane;
+                        SmVec<SPL> o00 = o[0], o01 = o[64], o10 = o[pw * 64], o11 = o[(pw + 1) * 64];
+#pragma unroll
+                        for (int j = 0; j < SPL; ++j) {
+                            float P = em * val[i].v[j];
+                            if constexpr (RATIO) P = em * (val[i].v[j] * rv[i].v[j]);
+                            o00.v[j] = fmaf(a00, P, o00.v[j]);
+                            o01.v[j] = fmaf(a01, P, o01.v[j]);
+                            o10.v[j] = fmaf(a10, P, o10.v[j]);
+                            o11.v[j] = fmaf(a11, P, o11.v[j]);
+                        }
+                        o[0] = o00;
+                        o[64] = o01;
+                        o[pw * 64] = o10;
+                        o[(pw + 1) * 64] = o11;
+                    }
+                }
+            }
+        }
+    }
+
+    // ---- the windows summed in wavefront order; entry q = across * 9 + along -
+    __syncthreads();
+    SmVec<SPL> *e = reinterpret_cast<SmVec<SPL> *>(p.edge + sm_edge_index(g, grp, it.side, it.t, it.d, it.ch) *
+                                                                kSmEdgeVox * S);
+    for (int q = wave; q < kSmEdgeVox; q += kSmEW) {
+        const int across = q / kSmWin, along = q % kSmWin;
+        const int v = rows ? across * kSmWin + along : along * 2 + across;
+        SmVec<SPL> x = win_all[v * 64 + lane];
+        for (int u = 1; u < kSmEW; ++u) {
+            const SmVec<SPL> y = win_all[(u * kSmEdgeVox + v) * 64 + lane];
+#pragma unroll
+            for (int j = 0; j < SPL; ++j) x.v[j] += y.v[j];
+        }
+        e[q * 64 + lane] = x;
+    }
+}
+
+// Sum of a strip's windows over its depth bins and angle ranges, in that fixed
+// order: one workgroup per (slice group, side, strip, window entry).
+template <int SPL>
+__global__ __launch_bounds__(64) void sm_edge_reduce_kernel(SmArgs p)
+{
+    constexpr int S = 64 * SPL;
+    const SmGeom &g = p.g;
+    const int ns = g.nsi > g.nsj ? g.nsi : g.nsj;
+    int id = blockIdx.x;
+    const int q = id % kSmEdgeVox;
+    id /= kSmEdgeVox;
+    const int t = id % ns;
+    id /= ns;
+    const int side = id % 4, grp = id / 4;
+    if (t >= (side < 2 ? g.nsj : g.nsi)) return;
+    const int lane = threadIdx.x;
+    const int n = g.nd * g.nch;
+    const SmVec<SPL> *src = reinterpret_cast<const SmVec<SPL> *>(p.edge) +
+                            (sm_edge_index(g, grp, side, t, 0, 0) * kSmEdgeVox + q) * 64 + lane;
+    SmVec<SPL> acc = src[0];
+    for (int i = 1; i < n; ++i) {
+        const SmVec<SPL> x = src[(int64_t)i * kSmEdgeVox * 64];
+#pragma unroll
+        for (int j = 0; j < SPL; ++j) acc.v[j] += x.v[j];
+    }
+    reinterpret_cast<SmVec<SPL> *>(p.edge_sum)[((((int64_t)grp * 4 + side) * ns + t) * kSmEdgeVox + q) * 64 + lane] =
+        acc;
+}
+
+// ---------------------------------------------------------------------------
+// Interior blocks.
+// ---------------------------------------------------------------------------
+template <int SPL, bool RATIO>
+__global__ __launch_bounds__(64) void sm_interior_kernel(SmArgs p)
+{
+    constexpr int S = 64 * SPL;
+    // cells whose loads are in flight: as many as the registers left at the
+    // occupancy the 25 * SPL accumulators allow
+    constexpr int RING = SPL == 2 ? 6 : 8;
+    // the MLEM ratio rows (detector columns) the block's points use at one angle
+    __shared__ SmVec<SPL> rat[kSmRatRows * 64];
+    const SmGeom &g = p.g;
+    const int nint = g.nbi * g.nbj;
+    const uint32_t id = xcd_remap(blockIdx.x, gridDim.x);
+    const int grp = (int)(id / (uint32_t)nint);
+    const int k = (int)(id % (uint32_t)nint);
+    const int bi = k / g.nbj, bj = k % g.nbj;
+    const int lane = threadIdx.x;
+    const int b0 = grp * S + lane * SPL;  // this lane's first slice (B is a multiple of SPL)
+    const float xmax = g.H - 1.001f, ymax = g.W - 1.001f;
+    const float cx = (g.W - 1) * 0.5f, cy = (g.H - 1) * 0.5f;
+    const int64_t HWB = (int64_t)g.H * g.W * g.B;
+    const int64_t WB = (int64_t)g.W * g.B;
+    const uint32_t voff = b0 < g.B ? (uint32_t)b0 * 4u : kSmOff;
+    // lane 4c + q: candidate q of cell c = (c / 4, c % 4) of the block
+    const int cell = lane >> 2, q = lane & 3;
+    const int Y = bi * kSmTB + (cell >> 2), X = bj * kSmTB + (cell & 3);
+    const float rr = (float)Y + 0.5f - cy, cc = (float)X + 0.5f - cx;
+
+    SmVec<SPL> acc[kSmWBVox];
+#pragma unroll
+    for (int v = 0; v < kSmWBVox; ++v)
+#pragma unroll
+        for (int j = 0; j < SPL; ++j) acc[v].v[j] = 0.0f;
+
+    for (int a = 0; a < g.A; ++a) {
+        const float th = finite_or_zero(scalar_read(p.theta, a));
+        const float c = cosf(-th), s = sinf(-th);
+        const float em = scalar_read(p.em, a);
+        const __amdgpu_buffer_rsrc_t ra = make_rsrc(p.atten + (int64_t)a * HWB, (int)HWB);
+
+        // ---- candidate q of this lane's cell: the lattice points of the
+        // bounding box of the cell's preimage (a rotated unit square), widened
+        // for rounding, are the 2 x 2 points from (ha, wa)
+        const float hs = fmaf(c, rr, s * cc) + cy, ws = fmaf(-s, rr, c * cc) + cx;
+        const float rad = 0.5f * (fabsf(c) + fabsf(s)) + 1e-3f;
+        const int ha = (int)ceilf(hs - rad), wa = (int)ceilf(ws - rad);
+        const int h = ha + (q >> 1), w = wa + (q & 1);
+        const float hc = (float)h - cy, wc = (float)w - cx;
+        const float xu = fmaf(c, hc, fmaf(-s, wc, cy));  // the forward's arithmetic
+        const float yu = fmaf(s, hc, fmaf(c, wc, cx));
+        const bool hit = h >= 0 && h < g.H && w >= 0 && w < g.W && xu >= 0.0f && xu <= xmax && yu >= 0.0f &&
+                         yu <= ymax && (int)xu == Y && (int)yu == X;
+        const int wmin = wave_min(wa);  // the block's candidates span fewer than kSmRatRows columns
+        const float fy = __builtin_amdgcn_fractf(xu), fx = __builtin_amdgcn_fractf(yu);
+        const float e1 = em * (1.0f - fy), e0 = em * fy;
+        uint32_t m_ao = hit ? (uint32_t)((h * g.W + w) * g.B) * 4u : kSmOff;
+        int m_rr = hit ? min(max(w - wmin, 0), kSmRatRows - 1) : 0;
+        float m00 = hit ? e1 * (1.0f - fx) : 0.0f, m01 = hit ? e1 * fx : 0.0f;
+        float m10 = hit ? e0 * (1.0f - fx) : 0.0f, m11 = hit ? e0 * fx : 0.0f;
+        // pack each cell's hits into its first lanes: lane 4c + r takes the r-th hit
+        const uint64_t hm = __ballot(hit);
+        const uint32_t qm = (uint32_t)(hm >> (4 * cell)) & 0xFu;
+        int src = -1, cnt = 0;
+#pragma unroll
+        for (int t = 0; t < 4; ++t) {
+            const bool set = (qm >> t) & 1u;
+            src = (set && cnt == q) ? t : src;
+            cnt += set ? 1 : 0;
+        }
+        const int from = 4 * (cell + 0) + (src < 0 ? q : src);
+        m_ao = (uint32_t)__shfl((int)m_ao, from);
+        m_rr = __shfl(m_rr, from);
+        m00 = __shfl(m00, from);
+        m01 = __shfl(m01, from);
+        m10 = __shfl(m10, from);
+        m11 = __shfl(m11, from);
+        if (src < 0) {
+            m_ao = kSmOff;
+            m00 = m01 = m10 = m11 = 0.0f;
+        }
+        const int npass = wave_max(cnt);
+
+        if constexpr (RATIO) {
+            const __amdgpu_buffer_rsrc_t rq = make_rsrc(p.ratio + (int64_t)a * WB, (int)WB);
+#pragma unroll
+            for (int r = 0; r < kSmRatRows; ++r) {
+                const int ww = wmin + r;
+                const uint32_t ro = (ww >= 0 && ww < g.W) ? voff + (uint32_t)(ww * g.B) * 4u : kSmOff;
+                rat[r * 64 + lane] = sm_load<SPL, 0>(rq, ro);
+            }
+        }
+
+        for (int ps = 0; ps < npass; ++ps) {
+            SmVec<SPL> va[RING], vr[RING];
+#pragma unroll
+            for (int i = 0; i < kSmTB * kSmTB + RING; ++i) {
+                // consume cell i - RING first: its ring slot is the one cell i's loads take
+                if (i >= RING) {
+                    const int cl = i - RING;
+                    const int L = 4 * cl + ps;
+                    SmVec<SPL> av = va[cl % RING];
+                    SmVec<SPL> rv = vr[cl % RING];
+#pragma unroll
+                    for (int j = 0; j < SPL; ++j) {
+                        // keep the product here: computed at load time it would make
+                        // every step wait for its own loads
+                        asm volatile("" : "+v"(av.v[j]), "+v"(rv.v[j]));
+                    }
+                    const float a00 = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(m00), L));
+                    const float a01 = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(m01), L));
+                    const float a10 = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(m10), L));
+                    const float a11 = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(m11), L));
+                    const int v = (cl >> 2) * kSmWB + (cl & 3);
+#pragma unroll
+                    for (int j = 0; j < SPL; ++j) {
+                        const float P = RATIO ? av.v[j] * rv.v[j] : av.v[j];
+                        acc[v].v[j] = fmaf(a00, P, acc[v].v[j]);
+                        acc[v + 1].v[j] = fmaf(a01, P, acc[v + 1].v[j]);
+                        acc[v + kSmWB].v[j] = fmaf(a10, P, acc[v + kSmWB].v[j]);
+                        acc[v + kSmWB + 1].v[j] = fmaf(a11, P, acc[v + kSmWB + 1].v[j]);
+                    }
+                }
+                if (i < kSmTB * kSmTB) {
+                    const int L = 4 * i + ps;
+                    // the point's offset in this lane's vector address (a VALU-written
+                    // SGPR read by a memory instruction would cost 5 wait states)
+                    va[i % RING] = sm_load<SPL, 2>(ra, voff + (uint32_t)__builtin_amdgcn_readlane((int)m_ao, L));
+                    if constexpr (RATIO) vr[i % RING] = rat[__builtin_amdgcn_readlane(m_rr, L) * 64 + lane];
+                }
+            }
+        }
+    }
+
+    // ---- the window: own 4 x 4 voxels, then the halo row and column ---------
+    const int HO = g.nbi * kSmTB, WO = g.nbj * kSmTB;
+    SmVec<SPL> *own = reinterpret_cast<SmVec<SPL> *>(p.own + (int64_t)grp * HO * WO * S);
+#pragma unroll
+    for (int yy = 0; yy < kSmTB; ++yy)
+#pragma unroll
+        for (int xx = 0; xx < kSmTB; ++xx)
+            own[((int64_t)(bi * kSmTB + yy) * WO + (bj * kSmTB + xx)) * 64 + lane] = acc[yy * kSmWB + xx];
+    SmVec<SPL> *hal = reinterpret_cast<SmVec<SPL> *>(p.halo + (((int64_t)grp * g.nbi + bi) * g.nbj + bj) * kSmHaloB * S);
+#pragma unroll
+    for (int xx = 0; xx < kSmWB; ++xx) hal[xx * 64 + lane] = acc[kSmTB * kSmWB + xx];
+#pragma unroll
+    for (int yy = 0; yy < kSmTB; ++yy) hal[(kSmWB + yy) * 64 + lane] = acc[yy * kSmWB + kSmTB];
+}
+
+// ---------------------------------------------------------------------------
+// Finalize: per voxel the fixed-order sum of every item's contribution, back to
+// (B, H, W), with the output mode applied.  One workgroup: one image row y,
+// 64 columns, 64 slices; first lane = slice, then lane = column.
+// ---------------------------------------------------------------------------
+struct SmFin {
+    SmGeom g;
+    const float *own, *halo, *edge_sum;
+    float *out;          // (B, H, W)
+    int mode;            // BwdMode
+    const float *C;      // update: (B, H, W)
+    const float *sens;   // update
+    float beta, delta;
+};
+
+__device__ __forceinline__ float sm_voxel_sum(const SmFin &f, int b, int y, int x)
+{
+    const SmGeom &g = f.g;
+    const int HO = g.nbi * kSmTB, WO = g.nbj * kSmTB;
+    const int gi = b / g.S, li = b % g.S;
+    const int bi = y / kSmTB, bj = x / kSmTB, ry = y % kSmTB, rx = x % kSmTB;
+    float v = 0.0f;
+    if (bi < g.nbi && bj < g.nbj) v = f.own[(((int64_t)gi * HO + y) * WO + x) * g.S + li];
+    const float *H0 = f.halo + (int64_t)gi * g.nbi * g.nbj * kSmHaloB * g.S;
+    // bottom halo of the block above: its window row 4
+    if (ry == 0 && bi >= 1 && bi - 1 < g.nbi) {
+        if (bj < g.nbj) v += H0[(((int64_t)(bi - 1) * g.nbj + bj) * kSmHaloB + rx) * g.S + li];
+        if (rx == 0 && bj >= 1 && bj - 1 < g.nbj)
+            v += H0[(((int64_t)(bi - 1) * g.nbj + bj - 1) * kSmHaloB + kSmTB) * g.S + li];
+    }
+    // right halo of the block to the left: its window column 4
+    if (rx == 0 && bj >= 1 && bj - 1 < g.nbj && bi < g.nbi)
+        v += H0[(((int64_t)bi * g.nbj + bj - 1) * kSmHaloB + kSmWB + ry) * g.S + li];
+    // edge strips covering the voxel, side by side
+    const int ns = g.nsi > g.nsj ? g.nsi : g.nsj;
+    for (int side = 0; side < 4; ++side) {
+        const bool rows = side < 2;
+        const int along = rows ? x : y;              // position along the border
+        const int across = rows ? (side == kSmTop ? y : y - (g.H - 2)) : (side == kSmLeft ? x : x - (g.W - 2));
+        if (across < 0 || across > 1) continue;
+        const int nstrip = rows ? g.nsj : g.nsi;
+        for (int dt = 0; dt < 2; ++dt) {
+            const int t = (along >> 3) - dt;
+            const int pos = along - 8 * t;           // 0..8 within the strip's window
+            if (t < 0 || t >= nstrip || pos > kSmT) continue;
+            const int qq = across * kSmWin + pos;
+            v += f.edge_sum[((((int64_t)gi * 4 + side) * ns + t) * kSmEdgeVox + qq) * g.S + li];
+        }
+    }
+    return v;
+}
+
+__global__ __launch_bounds__(64) void sm_finalize_kernel(SmFin f)
+{
+    __shared__ float tile[64 * 65];
+    const SmGeom &g = f.g;
+    const int x0 = blockIdx.x * 64, y = blockIdx.y, c64 = blockIdx.z;
+    const int lane = threadIdx.x;
+    // phase 1: lane = slice; voxels x0..x0+63 of row y
+    const int nx = min(64, g.W - x0);
+    const int bl = c64 * 64 + lane;
+    if (bl < g.B)
+        for (int i = 0; i < nx; ++i) tile[lane * 65 + i] = sm_voxel_sum(f, bl, y, x0 + i);
+    __syncthreads();
+    // phase 2: lane = column
+    const int x = x0 + lane;
+    if (x >= g.W) return;
+    const int64_t HW = (int64_t)g.H * g.W;
+    const int nb = min(64, g.B - c64 * 64);
+    for (int sl = 0; sl < nb; ++sl) {
+        const int bb = c64 * 64 + sl;
+        const float back = tile[sl * 65 + lane];
+        const int64_t o = (int64_t)bb * HW + (int64_t)y * g.W + x;
+        if (f.mode == kBwdWrite) {
+            f.out[o] = back;
+        } else if (f.mode == kBwdAccumulate) {
+            f.out[o] += back;
+        } else if (f.mode == kBwdSens) {
+            f.out[o] = back < 1e-6f ? 1e-6f : back;  // sens.clamp_min_(1e-6)
+        } else {
+            // C.mul_(back).div_(sens + beta*grad); C.clamp_min_(0)  (atten_cuda.cpp:334-341)
+            const float Cv = f.C[o];
+            float den = f.sens[o];
+            if (f.beta > 0.0f) den = den + f.beta * huber_at(f.C + (int64_t)bb * HW, y, x, g.H, g.W, f.delta);
+            const float v = (Cv * back) / den;
+            f.out[o] = v < 0.0f ? 0.0f : v;
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// (N, B, L) -> (N, L, B): the slice-major copy.  64 x 64 tiles through LDS.
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void sm_transpose_kernel(const float *src, float *dst, int B, int64_t L)
+{
+    __shared__ float tile[64 * 65];
+    const int64_t l0 = (int64_t)blockIdx.x * 64;
+    const int b0 = blockIdx.y * 64;
+    const int64_t n = blockIdx.z;
+    const float *s = src + n * (int64_t)B * L;
+    float *d = dst + n * (int64_t)B * L;
+    const int tx = threadIdx.x & 63, ty = threadIdx.x >> 6;
+    for (int r = ty; r < 64; r += 4) {
+        const int bb = b0 + r;
+        const int64_t l = l0 + tx;
+        tile[r * 65 + tx] = (bb < B && l < L) ? s[(int64_t)bb * L + l] : 0.0f;
+    }
+    __syncthreads();
+    for (int r = ty; r < 64; r += 4) {
+        const int64_t l = l0 + r;
+        const int bb = b0 + tx;
+        if (l < L && bb < B) d[l * B + bb] = tile[tx * 65 + r];
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Host side
+// ---------------------------------------------------------------------------
+SmGeom sm_geometry(int64_t A, int64_t B, int64_t H, int64_t W)
+{
+    SmGeom g{};
+    g.A = (int)A;
+    g.B = (int)B;
+    g.H = (int)H;
+    g.W = (int)W;
+    g.nsi = (int)((H - 1 + kSmT - 1) / kSmT);
+    g.nsj = (int)((W - 1 + kSmT - 1) / kSmT);
+    const double cx = (W - 1) * 0.5, cy = (H - 1) * 0.5;
+    const double reach = std::sqrt(cx * cx + cy * cy) + 2.0;
+    g.reach = (float)reach;
+    // deepest sample beyond the clamp box, in bins of 8 (the last bin is open)
+    g.nd = (int)std::ceil((reach - std::min(cx, cy) + 2.0) / 8.0) + 1;
+    g.nch = (int)std::max<int64_t>(1, std::min<int64_t>(8, A / 16));
+    // slices per lane: as many as the slices fill
+    g.spl = (B % 4 == 0 && B >= 192) ? 4 : (B % 2 == 0 && B >= 96) ? 2 : 1;
+    g.S = 64 * g.spl;
+    g.Gi = (int)((B + g.S - 1) / g.S);
+    g.nbi = (int)((H - 1 + kSmTB - 1) / kSmTB);
+    g.nbj = (int)((W - 1 + kSmTB - 1) / kSmTB);
+    return g;
+}
+
+// Scratch regions in floats: block sums, block halos, edge windows, their sums.
+static void sm_sizes(const SmGeom &g, size_t n[4])
+{
+    const int ns = std::max(g.nsi, g.nsj);
+    n[0] = (size_t)g.Gi * g.nbi * kSmTB * g.nbj * kSmTB * g.S;
+    n[1] = (size_t)g.Gi * g.nbi * g.nbj * kSmHaloB * g.S;
+    n[2] = (size_t)g.Gi * 4 * ns * g.nd * g.nch * kSmEdgeVox * g.S;
+    n[3] = (size_t)g.Gi * 4 * ns * kSmEdgeVox * g.S;
+}
+
+static size_t sm_align(size_t floats) { return (floats * sizeof(float) + 255) / 256 * 256; }
+
+size_t sm_scratch_bytes(int64_t A, int64_t B, int64_t H, int64_t W)
+{
+    const SmGeom g = sm_geometry(A, B, H, W);
+    size_t n[4];
+    sm_sizes(g, n);
+    return sm_align(n[0]) + sm_align(n[1]) + sm_align(n[2]) + sm_align(n[3]);
+}
+
+int launch_sm_transpose(const float *src, float *dst, int64_t N, int64_t B, int64_t L, hipStream_t s)
+{
+    if (N == 0 || B == 0 || L == 0) return FL_OK;
+    FL_REQUIRE(N < 65536 && (L + 63) / 64 < (int64_t)UINT32_MAX, "slice-major copy: shape too large");
+    sm_transpose_kernel<<<dim3((uint32_t)((L + 63) / 64), (uint32_t)((B + 63) / 64), (uint32_t)N), dim3(256), 0,
+                          s>>>(src, dst, (int)B, L);
+    FL_LAUNCH_CHECK();
+    return FL_OK;
+}
+
+template <int SPL>
+static void sm_launch(const SmArgs &p, uint32_t n_int, uint32_t n_edge, uint32_t n_red, hipStream_t stream)
+{
+    if (p.ratio) {
+        sm_interior_kernel<SPL, true><<<dim3(n_int), dim3(64), 0, stream>>>(p);
+        sm_edge_kernel<SPL, true><<<dim3(n_edge), dim3(64 * kSmEW), 0, stream>>>(p);
+    } else {
+        sm_interior_kernel<SPL, false><<<dim3(n_int), dim3(64), 0, stream>>>(p);
+        sm_edge_kernel<SPL, false><<<dim3(n_edge), dim3(64 * kSmEW), 0, stream>>>(p);
+    }
+    sm_edge_reduce_kernel<SPL><<<dim3(n_red), dim3(64), 0, stream>>>(p);
+}
+
+int launch_sm_backward(const float *atten_sm, const float *ratio_sm, const float *em, const float *theta,
+                       float *out, int64_t A, int64_t B, int64_t H, int64_t W, int mode, const float *C,
+                       const float *sens, float beta, float delta, void *scratch, size_t scratch_bytes,
+                       hipStream_t stream)
+{
+    const SmGeom g = sm_geometry(A, B, H, W);
+    FL_REQUIRE(scratch_bytes >= sm_scratch_bytes(A, B, H, W) && scratch, "slice-major back-projector: scratch too small");
+    FL_REQUIRE(H * W * B < ((int64_t)1 << 29), "slice-major back-projector: H*W*B must be below 2^29");
+    size_t n[4];
+    sm_sizes(g, n);
+    char *ws = (char *)scratch;
+    SmArgs p{};
+    p.g = g;
+    p.atten = atten_sm;
+    p.ratio = ratio_sm;
+    p.em = em;
+    p.theta = theta;
+    p.own = (float *)ws;
+    p.halo = (float *)(ws + sm_align(n[0]));
+    p.edge = (float *)(ws + sm_align(n[0]) + sm_align(n[1]));
+    p.edge_sum = (float *)(ws + sm_align(n[0]) + sm_align(n[1]) + sm_align(n[2]));
+    const int64_t n_int = (int64_t)g.Gi * g.nbi * g.nbj;
+    const int64_t n_e = (int64_t)g.Gi * sm_edge_items(g);
+    const int64_t n_red = (int64_t)g.Gi * 4 * std::max(g.nsi, g.nsj) * kSmEdgeVox;
+    FL_REQUIRE(n_int < (int64_t)UINT32_MAX && n_e < (int64_t)UINT32_MAX && n_red < (int64_t)UINT32_MAX,
+               "slice-major back-projector: grid too large");
+    if (g.spl == 4)
+        sm_launch<4>(p, (uint32_t)n_int, (uint32_t)n_e, (uint32_t)n_red, stream);
+    else if (g.spl == 2)
+        sm_launch<2>(p, (uint32_t)n_int, (uint32_t)n_e, (uint32_t)n_red, stream);
+    else
+        sm_launch<1>(p, (uint32_t)n_int, (uint32_t)n_e, (uint32_t)n_red, stream);
+    FL_LAUNCH_CHECK();
+    SmFin f{};
+    f.g = g;
+    f.own = p.own;
+    f.halo = p.halo;
+    f.edge_sum = p.edge_sum;
+    f.out = out;
+    f.mode = mode;
+    f.C = C;
+    f.sens = sens;
+    f.beta = beta;
+    f.delta = delta;
+    sm_finalize_kernel<<<dim3((uint32_t)((W + 63) / 64), (uint32_t)H, (uint32_t)((B + 63) / 64)), dim3(64), 0,
+                         stream>>>(f);
+    FL_LAUNCH_CHECK();
+    return FL_OK;
+}
+
+}  // namespace fl

@@ -2,7 +2,7 @@
 // its arguments on the host, then enqueues kernels on the caller's stream.
 // No entry point allocates device memory: scratch comes from the caller's
 // workspace argument, or for fl_mlem_batch_noalloc from a static device array.
-#include "common.hpp"
+#include "projector.hpp"
 
 #include <mutex>
 #include <string>
@@ -13,7 +13,8 @@ static thread_local std::string g_last_error;
 void set_error(const std::string &msg) { g_last_error = msg; }
 
 int launch_forward(const float *atten, const float *C, const float *em, const float *theta, const float *I,
-                   float *out, int64_t A, int64_t B, int64_t R, int64_t H, int64_t W, int mode, hipStream_t stream);
+                   float *out, int64_t A, int64_t B, int64_t R, int64_t H, int64_t W, int mode, hipStream_t stream,
+                   int out_sm = 0);
 int launch_backward(const float *atten, const float *ratio, const float *em, const float *theta,
                     float *out, int64_t A, int64_t B, int64_t R, int64_t H, int64_t W, int mode,
                     const float *C, const float *sens, float beta, float delta, hipStream_t stream);
@@ -87,6 +88,10 @@ static int scratch_address(int dev, void **out)
 }  // namespace fl
 
 using namespace fl;
+
+// Slices at which fl_mlem_batch takes the slice-major path (64 slices per
+// wavefront there: with fewer, most lanes would idle).
+constexpr int64_t kSmMinSlices = 32;
 
 extern "C" {
 
@@ -185,6 +190,89 @@ int fl_mlem_update(const float *atten, const float *ratio, const float *em_cs, c
                            (float)delta, (hipStream_t)stream);
 }
 
+// ---------------------------------------------------------------------------
+// Slice-major path: the back-projector on atten_sm (A, H, W, B)
+// ---------------------------------------------------------------------------
+
+int fl_slice_major(const float *src, float *dst, int64_t N, int64_t B, int64_t L, void *stream)
+{
+    g_last_error.clear();
+    FL_REQUIRE(N >= 0 && B >= 0 && L >= 0, "shape: negative size");
+    if (N == 0 || B == 0 || L == 0) return FL_OK;
+    FL_REQUIRE(src && dst && src != dst, "null or aliased pointer argument");
+    return launch_sm_transpose(src, dst, N, B, L, (hipStream_t)stream);
+}
+
+size_t fl_backproject_sm_scratch_bytes(int64_t A, int64_t B, int64_t H, int64_t W)
+{
+    if (A < 1 || B < 1 || H < 2 || W < 2) return 0;
+    return sm_scratch_bytes(A, B, H, W);
+}
+
+static int check_sm(int64_t A, int64_t B, int64_t H, int64_t W)
+{
+    if (int e = check_dims(A, B, 1, H, W)) return e;
+    FL_REQUIRE(A >= 1, "at least one angle");
+    FL_REQUIRE(H * W * B < ((int64_t)1 << 29), "slice-major path: H*W*B must be below 2^29");
+    return FL_OK;
+}
+
+int fl_mlem_ratio_sm(const float *atten, const float *C, const float *em_cs, const float *theta, const float *I,
+                     float *ratio_sm, int64_t A, int64_t B, int64_t H, int64_t W, void *stream)
+{
+    g_last_error.clear();
+    if (int e = check_sm(A, B, H, W)) return e;
+    if (B == 0) return FL_OK;
+    FL_REQUIRE(atten && C && em_cs && theta && I && ratio_sm, "null pointer argument");
+    return launch_forward(atten, C, em_cs, theta, I, ratio_sm, A, B, 1, H, W, 2, (hipStream_t)stream, 1);
+}
+
+int fl_backward_emission_sm(const float *atten_sm, const float *ratio_sm, const float *em_cs, const float *theta,
+                            float *out, int64_t A, int64_t B, int64_t H, int64_t W, int accumulate, void *scratch,
+                            size_t scratch_bytes, void *stream)
+{
+    g_last_error.clear();
+    if (int e = check_sm(A, B, H, W)) return e;
+    if (B == 0) return FL_OK;
+    FL_REQUIRE(atten_sm && em_cs && theta && out, "null pointer argument");
+    return launch_sm_backward(atten_sm, ratio_sm, em_cs, theta, out, A, B, H, W,
+                              accumulate ? kBwdAccumulate : kBwdWrite, nullptr, nullptr, 0.f, 0.f, scratch,
+                              scratch_bytes, (hipStream_t)stream);
+}
+
+int fl_mlem_sensitivity_sm(const float *atten_sm, const float *em_cs, const float *theta, float *sens, int64_t A,
+                           int64_t B, int64_t H, int64_t W, void *scratch, size_t scratch_bytes, void *stream)
+{
+    g_last_error.clear();
+    if (int e = check_sm(A, B, H, W)) return e;
+    if (B == 0) return FL_OK;
+    FL_REQUIRE(atten_sm && em_cs && theta && sens, "null pointer argument");
+    return launch_sm_backward(atten_sm, nullptr, em_cs, theta, sens, A, B, H, W, kBwdSens, nullptr, nullptr, 0.f,
+                              0.f, scratch, scratch_bytes, (hipStream_t)stream);
+}
+
+int fl_mlem_update_sm(const float *atten_sm, const float *ratio_sm, const float *em_cs, const float *theta,
+                      const float *C, const float *sens, float *C_next, int64_t A, int64_t B, int64_t H, int64_t W,
+                      double beta, double delta, void *scratch, size_t scratch_bytes, void *stream)
+{
+    g_last_error.clear();
+    if (int e = check_sm(A, B, H, W)) return e;
+    if (B == 0) return FL_OK;
+    FL_REQUIRE(atten_sm && ratio_sm && em_cs && theta && C && sens && C_next, "null pointer argument");
+    FL_REQUIRE(C_next != C, "C_next must not alias C (neighbours of C are read by other workgroups)");
+    return launch_sm_backward(atten_sm, ratio_sm, em_cs, theta, C_next, A, B, H, W, kBwdUpdate, C, sens,
+                              (float)beta, (float)delta, scratch, scratch_bytes, (hipStream_t)stream);
+}
+
+size_t fl_mlem_workspace_bytes_sm(int64_t A, int64_t B, int64_t R, int64_t H, int64_t W)
+{
+    if (R != 1 || A < 1 || B < 1 || H < 2 || W < 2 || H * W * B >= ((int64_t)1 << 29)) return 0;
+    auto al = [](size_t f) { return (f * sizeof(float) + 255) / 256 * 256; };
+    const size_t nC = (size_t)(B * H * W), nP = (size_t)(A * B * W);
+    // sens, ping-pong estimate, ratio (A, W, B), atten_sm (A, H, W, B), back-projector scratch
+    return al(nC) + al(nC) + al(nP) + al((size_t)A * nC) + sm_scratch_bytes(A, B, H, W);
+}
+
 size_t fl_mlem_workspace_bytes(int64_t A, int64_t B, int64_t R, int64_t H, int64_t W)
 {
     const size_t nC = (size_t)(R * B * H * W);
@@ -217,6 +305,31 @@ int fl_mlem_batch(const float *C_init, const float *atten, const float *em_cs,
     float *sens = (float *)ws;
     float *other = (float *)(ws + al(nC));
     float *ratio = (float *)(ws + 2 * al(nC));
+    const size_t need_sm = fl_mlem_workspace_bytes_sm(A, B, R, H, W);
+    if (need_sm && workspace_bytes >= need_sm && B >= kSmMinSlices) {
+        // slice-major path: a slice-major copy of atten once, then every
+        // back-projection on it (the forward keeps the reference's layout)
+        const size_t nP = (size_t)(A * B * W);
+        float *atten_sm = (float *)(ws + 2 * al(nC) + al(nP));
+        void *scr = ws + 2 * al(nC) + al(nP) + al((size_t)A * nC);
+        const size_t scr_bytes = sm_scratch_bytes(A, B, H, W);
+        if (int e = fl_slice_major(atten, atten_sm, A, B, H * W, stream)) return e;
+        if (int e = fl_mlem_sensitivity_sm(atten_sm, em_cs, theta, sens, A, B, H, W, scr, scr_bytes, stream)) return e;
+        FL_HIP_CHECK(hipMemcpyAsync(C_out, C_init, nC * sizeof(float), hipMemcpyDeviceToDevice, s));
+        float *cur = C_out, *nxt = other;
+        for (int64_t it = 0; it < n_iter; ++it) {
+            if (int e = fl_mlem_ratio_sm(atten, cur, em_cs, theta, I, ratio, A, B, H, W, stream)) return e;
+            if (int e = fl_mlem_update_sm(atten_sm, ratio, em_cs, theta, cur, sens, nxt, A, B, H, W, beta, delta,
+                                          scr, scr_bytes, stream))
+                return e;
+            float *tmp = cur;
+            cur = nxt;
+            nxt = tmp;
+        }
+        if (cur != C_out)
+            FL_HIP_CHECK(hipMemcpyAsync(C_out, cur, nC * sizeof(float), hipMemcpyDeviceToDevice, s));
+        return FL_OK;
+    }
 
     if (int e = fl_mlem_sensitivity(atten, em_cs, theta, sens, A, B, R, H, W, stream)) return e;
     // C = C_init.clone()  (atten_cuda.cpp:303); iterate between C_out and `other`
@@ -264,8 +377,9 @@ int fl_mlem_batch_noalloc(const float *C_init, const float *atten, const float *
     // every region of the scratch is written before it is read (sens by the sensitivity
     // pass, the ratio by each forward pass, the second estimate by each update), so the
     // previous call's contents need no clearing
-    return fl_mlem_batch(C_init, atten, em_cs, theta, I, C_out, A, B, R, H, W, n_iter, beta, delta, ws,
-                         (size_t)FL_MLEM_NOALLOC_SCRATCH_BYTES, stream);
+    // exactly the workspace fl_mlem_workspace_bytes asks for: the same kernels and
+    // results as fl_mlem_batch called with that workspace
+    return fl_mlem_batch(C_init, atten, em_cs, theta, I, C_out, A, B, R, H, W, n_iter, beta, delta, ws, need, stream);
 }
 
 // ---------------------------------------------------------------------------

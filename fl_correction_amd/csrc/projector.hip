@@ -250,12 +250,14 @@ __device__ __forceinline__ void fwd_strip_tile(const FwdArgs &p, uint32_t t)
         const float *rk = red + k * kFWaves * 64 + l;
         float tot = ((rk[0] + rk[64]) + rk[128]) + rk[192];
         if (p.nref == 1) tot *= emv[k * kFRefChunk];
-        const int64_t o = ((int64_t)(a0 + k) * p.B + b) * p.W + wo;
+        // (A, B, W), or slice-major (A, W, B) for the slice-major back-projector
+        const int64_t o = p.out_sm ? ((int64_t)(a0 + k) * p.W + wo) * p.B + b : ((int64_t)(a0 + k) * p.B + b) * p.W + wo;
+        const int64_t oi = ((int64_t)(a0 + k) * p.B + b) * p.W + wo;
         if (p.mode == kFwdAccumulate || p.mode == kFwdAccumulateRatio) tot += p.out[o];
         if (p.mode >= kFwdRatio) {
             // Pf.clamp_min_(1e-6); ratio = I / Pf   (atten_cuda.cpp:326-328)
             const float pf = tot < 1e-6f ? 1e-6f : tot;
-            tot = p.I[o] / pf;
+            tot = p.I[oi] / pf;
         }
         p.out[o] = tot;
     }
@@ -270,23 +272,6 @@ __global__ __launch_bounds__(256, 2) void fwd_strip_kernel(FwdArgs p)
 // ---------------------------------------------------------------------------
 // Back-projection (exact transpose) and the fused MLEM update
 // ---------------------------------------------------------------------------
-
-__device__ __forceinline__ float huber_at(const float *Cs, int i, int j, int H, int W, float delta)
-{
-    // FL/cuda_lib/atten_cuda_kernel.cu:593-613: neighbours up, down, left, right
-    const float Cp = Cs[(int64_t)i * W + j];
-    float g = 0.0f;
-    const int di[4] = {-1, 1, 0, 0};
-    const int dj[4] = {0, 0, -1, 1};
-#pragma unroll
-    for (int k = 0; k < 4; ++k) {
-        const int ii = i + di[k], jj = j + dj[k];
-        if (ii < 0 || ii >= H || jj < 0 || jj >= W) continue;
-        const float diff = Cp - Cs[(int64_t)ii * W + jj];
-        g += fabsf(diff) <= delta ? diff : delta * copysignf(1.0f, diff);
-    }
-    return g;
-}
 
 // Range of qc with lo <= fmaf(slope, qc, base) < hi, intersected into [qmin, qmax].
 // The real-arithmetic bounds are widened by 0.01 plus the distance the rounding
@@ -1069,7 +1054,8 @@ __global__ __launch_bounds__(256) void huber_kernel(const float *C, float *grad,
 
 // mode: kFwdWrite (Pf) or kFwdRatio (ratio = I / max(Pf, 1e-6))
 int launch_forward(const float *atten, const float *C, const float *em, const float *theta, const float *I,
-                   float *out, int64_t A, int64_t B, int64_t R, int64_t H, int64_t W, int mode, hipStream_t stream)
+                   float *out, int64_t A, int64_t B, int64_t R, int64_t H, int64_t W, int mode, hipStream_t stream,
+                   int out_sm)
 {
     static bool attr_set = false;
     const size_t shmem1 = fwd_lds_bytes(1), shmem2 = fwd_lds_bytes(2);
@@ -1094,6 +1080,8 @@ int launch_forward(const float *atten, const float *C, const float *em, const fl
     p.W = (int)W;
     p.ntw = (int)((W + kFW - 1) / kFW);
     p.n_groups = (int)((A + kFG - 1) / kFG);
+    p.out_sm = out_sm;
+    FL_REQUIRE(!out_sm || R <= kFRefChunk, "slice-major output: at most two references");
     const int64_t nwg = (int64_t)p.ntw * p.n_groups * B;
     FL_REQUIRE(nwg < (int64_t)UINT32_MAX, "forward projector: grid too large");
     for (int64_t r0 = 0; r0 < R; r0 += kFRefChunk) {

@@ -45,6 +45,7 @@ struct FwdArgs {
     int mode;
     int ntw, n_groups;
     int group_angles;  // grouped launches: angles per workgroup of this job (4 or 8)
+    int out_sm;        // write out as (A, W, B) instead of (A, B, W)
 };
 
 // ---------------------------------------------------------------------------
@@ -122,5 +123,33 @@ int launch_bwd_group(const BwdArgs *jobs, const uint32_t *begin, int n_jobs, uin
                      hipStream_t s);
 int launch_copy_group(const CopyJob *jobs, const uint32_t *begin, int n_jobs, uint32_t n_tiles, int grid,
                       hipStream_t s);
+
+// 4-neighbour Huber gradient of C at (i, j) in one (H, W) plane
+// (FL/cuda_lib/atten_cuda_kernel.cu:593-613: neighbours up, down, left, right).
+__device__ __forceinline__ float huber_at(const float *Cs, int i, int j, int H, int W, float delta)
+{
+    const float Cp = Cs[(int64_t)i * W + j];
+    float g = 0.0f;
+    const int di[4] = {-1, 1, 0, 0};
+    const int dj[4] = {0, 0, -1, 1};
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        const int ii = i + di[k], jj = j + dj[k];
+        if (ii < 0 || ii >= H || jj < 0 || jj >= W) continue;
+        const float diff = Cp - Cs[(int64_t)ii * W + jj];
+        g += fabsf(diff) <= delta ? diff : delta * copysignf(1.0f, diff);
+    }
+    return g;
+}
+
+// Slice-major back-projector (bwd_sm.hip): atten_sm (A, H, W, B), ratio_sm (A, W, B)
+// or null, one reference per line (em (A)); out/C/sens (B, H, W); mode a BwdMode.
+size_t sm_scratch_bytes(int64_t A, int64_t B, int64_t H, int64_t W);
+int launch_sm_backward(const float *atten_sm, const float *ratio_sm, const float *em, const float *theta,
+                       float *out, int64_t A, int64_t B, int64_t H, int64_t W, int mode, const float *C,
+                       const float *sens, float beta, float delta, void *scratch, size_t scratch_bytes,
+                       hipStream_t stream);
+// (N, B, L) -> (N, L, B)
+int launch_sm_transpose(const float *src, float *dst, int64_t N, int64_t B, int64_t L, hipStream_t s);
 
 }  // namespace fl

@@ -112,6 +112,48 @@ FL_API int fl_mlem_batch_noalloc(const float *C_init, const float *atten, const 
                                  int64_t B, int64_t R, int64_t H, int64_t W, int64_t n_iter,
                                  double beta, double delta, void *stream);
 
+/* ---------------------------------------------------------------------------
+ * Slice-major path.  The back-projection reads the attenuation as a slice-major
+ * copy atten_sm (A, H, W, B): the B slices of one detector point are contiguous,
+ * so one sample position and its four weights serve 64 slices at once.  The
+ * results equal the reference-layout back-projector's to rounding (the sums run
+ * in another fixed order).  One reference per line (R = 1; em_cs (A)); ratio_sm
+ * is the MLEM ratio laid out (A, W, B); H*W*B < 2^29.  The back-projector needs
+ * fl_backproject_sm_scratch_bytes of device scratch (per-block partial sums).
+ * No reference counterpart for the layout: these compute backward_emission_batch
+ * (FL/cuda_lib/atten_cuda.cpp:188-214) and the MLEM stages of mlem_cuda_batch
+ * (atten_cuda.cpp:308-341) on it.
+ * ------------------------------------------------------------------------- */
+
+/* dst[n, l, b] = src[n, b, l] for n < N, b < B, l < L: atten (A,B,H,W) -> atten_sm
+ * with N = A, L = H*W. */
+FL_API int fl_slice_major(const float *src, float *dst, int64_t N, int64_t B, int64_t L, void *stream);
+FL_API size_t fl_backproject_sm_scratch_bytes(int64_t A, int64_t B, int64_t H, int64_t W);
+/* out (B,H,W) (=, or += with accumulate) = B^T ratio on atten_sm; ratio_sm NULL means 1. */
+FL_API int fl_backward_emission_sm(const float *atten_sm, const float *ratio_sm, const float *em_cs,
+                                   const float *theta, float *out, int64_t A, int64_t B, int64_t H,
+                                   int64_t W, int accumulate, void *scratch, size_t scratch_bytes,
+                                   void *stream);
+/* ratio_sm (A,W,B) = I / max(F C, 1e-6): the forward on atten (A,B,H,W), C (B,H,W), I (A,B,W). */
+FL_API int fl_mlem_ratio_sm(const float *atten, const float *C, const float *em_cs,
+                            const float *theta, const float *I, float *ratio_sm, int64_t A, int64_t B,
+                            int64_t H, int64_t W, void *stream);
+/* sens (B,H,W) = max(B^T 1, 1e-6) on atten_sm. */
+FL_API int fl_mlem_sensitivity_sm(const float *atten_sm, const float *em_cs, const float *theta,
+                                  float *sens, int64_t A, int64_t B, int64_t H, int64_t W,
+                                  void *scratch, size_t scratch_bytes, void *stream);
+/* C_next = max(0, C*(B^T ratio)/(sens + beta*huber(C))), or C*(B^T ratio)/sens when
+ * beta <= 0, on atten_sm and ratio_sm; C, sens, C_next (B,H,W); C_next must not alias C. */
+FL_API int fl_mlem_update_sm(const float *atten_sm, const float *ratio_sm, const float *em_cs,
+                             const float *theta, const float *C, const float *sens, float *C_next,
+                             int64_t A, int64_t B, int64_t H, int64_t W, double beta, double delta,
+                             void *scratch, size_t scratch_bytes, void *stream);
+/* Workspace with which fl_mlem_batch takes the slice-major path (0 if it does not
+ * apply: R != 1 or H*W*B too large).  fl_mlem_batch takes it when R = 1, B >= 32 and
+ * workspace_bytes is at least this (it then makes the slice-major copy itself); with
+ * fl_mlem_workspace_bytes it keeps the reference-layout back-projector. */
+FL_API size_t fl_mlem_workspace_bytes_sm(int64_t A, int64_t B, int64_t R, int64_t H, int64_t W);
+
 /* The three stages of fl_mlem_batch, for callers that drive the iteration
  * themselves (and time its kernels):
  *   fl_mlem_sensitivity: sens = max(B^T 1, 1e-6)                  (atten_cuda.cpp:308-314)
