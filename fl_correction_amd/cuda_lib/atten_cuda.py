@@ -195,7 +195,12 @@ def mlem_cuda_batch(C_init, atten, em_cs, theta, I, n_iter, beta=1e-3, delta=0.0
     dev = _same_device(C_init, atten, em_cs, theta, I)
     out = torch.empty_like(C_init)
     with torch.cuda.device(dev):
+        # the slice-major back-projection when its copy of atten fits in free
+        # memory (fl_mlem_batch takes it when handed a workspace that large)
+        from ..mlem import slice_major_fits
         nbytes = _lib.load().fl_mlem_workspace_bytes(A, B, R, H, W)
+        if slice_major_fits(A, B, R, H, W, dev, extra_bytes=nbytes):
+            nbytes = _lib.load().fl_mlem_workspace_bytes_sm(A, B, R, H, W)
         ws = _workspace(nbytes, dev)
         _lib.call("fl_mlem_batch", _lib.ptr(C_init), _lib.ptr(atten), _lib.ptr(em_cs), _lib.ptr(theta), _lib.ptr(I),
                   _lib.ptr(out), A, B, R, H, W, n_iter, float(beta), float(delta), _lib.ptr(ws), ws.numel() * 4,

@@ -174,3 +174,41 @@ def test_adjoint_identity_sm():
     rhs = float((C[0].double() * Bt.double()).sum())
     print(f"adjoint identity: {lhs:.9e} vs {rhs:.9e}")
     assert abs(lhs - rhs) <= 2e-6 * float((Pf.double().abs() * r.double().abs()).sum())
+
+
+@pytest.mark.parametrize("beta", [0.0, 1e-3])
+def test_mlem_solver_layouts_agree(beta):
+    """MLEMSolver on the slice-major layout against the reference layout and the oracle."""
+    from fl_correction_amd.mlem import MLEMSolver
+    A, B, H, W = 30, 48, 44, 52
+    rng = np.random.default_rng(21)
+    atten = rng.uniform(0.3, 1.0, (A, B, H, W)).astype(np.float32)
+    C_true = rng.uniform(0, 1, (1, B, H, W)).astype(np.float32)
+    em = rng.uniform(0.5, 2.0, (A, 1)).astype(np.float32)
+    theta = np.sort(rng.uniform(0, 2 * np.pi, A)).astype(np.float32)
+    I = oracle.forward_batch_f32(atten, C_true, em, theta)
+    at, emd, thd, Id = _dev(atten), _dev(em), _dev(theta), _dev(I)
+    C0 = torch.ones((1, B, H, W), device="cuda")
+    out = {}
+    for layout in ("slice_major", "reference"):
+        sv = MLEMSolver(at, emd, thd, Id, beta=beta, layout=layout)
+        assert sv.layout == layout
+        out[layout] = sv.run(C0, 6).cpu().numpy()
+    ref = oracle.mlem_batch_f32(np.ones((1, B, H, W), np.float32), atten, em, theta, I, 6, beta, 0.01)
+    print(f"solver beta={beta}: slice-major vs oracle {rel_l2(out['slice_major'], ref):.2e}, "
+          f"vs reference layout {rel_l2(out['slice_major'], out['reference']):.2e}")
+    assert rel_l2(out["slice_major"], ref) <= RTOL
+    assert rel_l2(out["slice_major"], out["reference"]) <= RTOL
+
+
+def test_mlem_solver_layout_choice():
+    from fl_correction_amd.mlem import MLEMSolver
+    A, H, W = 4, 16, 16
+    th = torch.zeros(A, device="cuda")
+    em = torch.ones((A, 1), device="cuda")
+    for B, want in ((8, "reference"), (40, "slice_major")):
+        at = torch.rand((A, B, H, W), device="cuda")
+        sv = MLEMSolver(at, em, th, torch.rand((A, B, W), device="cuda"))
+        assert sv.layout == want
+    with pytest.raises(ValueError):
+        MLEMSolver(at, em, th, torch.rand((A, 40, W), device="cuda"), layout="columns")
