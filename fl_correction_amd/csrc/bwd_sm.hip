@@ -56,7 +56,6 @@ constexpr int kSmTB = 4;                 // interior block: 4 x 4 base cells
 constexpr int kSmWB = kSmTB + 1;         // interior window side
 constexpr int kSmWBVox = kSmWB * kSmWB;  // 25
 constexpr int kSmHaloB = 2 * kSmTB + 1;  // halo of a block: row of 5, column of 4
-constexpr int kSmRatRows = 12;           // ratio rows cached per interior block and angle
 constexpr uint32_t kSmOff = 0x7FFFFFF0u; // an offset past every buffer (reads 0)
 
 enum SmKind { kSmTop = 0, kSmBottom = 1, kSmLeft = 2, kSmRight = 3 };
@@ -402,15 +401,55 @@ __global__ __launch_bounds__(64) void sm_edge_reduce_kernel(SmArgs p)
 // ---------------------------------------------------------------------------
 // Interior blocks.
 // ---------------------------------------------------------------------------
+// acc[v] += w * P for the four window voxels of base cell C (row C / 4, column
+// C % 4 of the block): register indices fixed at compile time.
+template <int C, int SPL>
+__device__ __forceinline__ void sm_acc_cell(SmVec<SPL> *acc, const SmVec<SPL> &P, float a00, float a01, float a10,
+                                            float a11)
+{
+    constexpr int v = (C >> 2) * kSmWB + (C & 3);
+#pragma unroll
+    for (int j = 0; j < SPL; ++j) {
+        acc[v].v[j] = fmaf(a00, P.v[j], acc[v].v[j]);
+        acc[v + 1].v[j] = fmaf(a01, P.v[j], acc[v + 1].v[j]);
+        acc[v + kSmWB].v[j] = fmaf(a10, P.v[j], acc[v + kSmWB].v[j]);
+        acc[v + kSmWB + 1].v[j] = fmaf(a11, P.v[j], acc[v + kSmWB + 1].v[j]);
+    }
+    // A marker that differs per cell: without it the compiler merges the 16
+    // cells' identical code into one block indexing acc with a variable, which
+    // moves acc out of registers into scratch memory.
+    asm volatile("; window cell %0" ::"n"(C));
+}
+
+// Binary tree of wave-uniform branches on the cell index (16 cells: 4 levels).
+template <int LO, int HI, int SPL>
+__device__ __forceinline__ void sm_acc_dispatch(int c, SmVec<SPL> *acc, const SmVec<SPL> &P, float a00, float a01,
+                                                float a10, float a11)
+{
+    if constexpr (LO == HI) {
+        sm_acc_cell<LO, SPL>(acc, P, a00, a01, a10, a11);
+    } else {
+        constexpr int MID = (LO + HI) / 2;
+        if (c <= MID)
+            sm_acc_dispatch<LO, MID, SPL>(c, acc, P, a00, a01, a10, a11);
+        else
+            sm_acc_dispatch<MID + 1, HI, SPL>(c, acc, P, a00, a01, a10, a11);
+    }
+}
+
+// Per-lane fields of one tested angle: the hit's atten and ratio byte offsets
+// and its four weights (the emission factor folded in).
+struct SmCand {
+    uint32_t ao, ro;
+    float w00, w01, w10, w11;
+};
+
 template <int SPL, bool RATIO>
 __global__ __launch_bounds__(64) void sm_interior_kernel(SmArgs p)
 {
     constexpr int S = 64 * SPL;
-    // cells whose loads are in flight: as many as the registers left at the
-    // occupancy the 25 * SPL accumulators allow
-    constexpr int RING = SPL == 2 ? 6 : 8;
-    // the MLEM ratio rows (detector columns) the block's points use at one angle
-    __shared__ SmVec<SPL> rat[kSmRatRows * 64];
+    // hits whose loads are in flight
+    constexpr int RING = 8;
     const SmGeom &g = p.g;
     const int nint = g.nbi * g.nbj;
     const uint32_t id = xcd_remap(blockIdx.x, gridDim.x);
@@ -435,15 +474,13 @@ __global__ __launch_bounds__(64) void sm_interior_kernel(SmArgs p)
 #pragma unroll
         for (int j = 0; j < SPL; ++j) acc[v].v[j] = 0.0f;
 
-    for (int a = 0; a < g.A; ++a) {
+    // Candidate q of this lane's cell at angle a: the lattice points of the
+    // bounding box of the cell's preimage (a rotated unit square), widened for
+    // rounding, are the 2 x 2 points from (ha, wa).  Returns the wave's hit mask.
+    auto test = [&](int a, SmCand &m) -> uint64_t {
         const float th = finite_or_zero(scalar_read(p.theta, a));
         const float c = cosf(-th), s = sinf(-th);
         const float em = scalar_read(p.em, a);
-        const __amdgpu_buffer_rsrc_t ra = make_rsrc(p.atten + (int64_t)a * HWB, (int)HWB);
-
-        // ---- candidate q of this lane's cell: the lattice points of the
-        // bounding box of the cell's preimage (a rotated unit square), widened
-        // for rounding, are the 2 x 2 points from (ha, wa)
         const float hs = fmaf(c, rr, s * cc) + cy, ws = fmaf(-s, rr, c * cc) + cx;
         const float rad = 0.5f * (fabsf(c) + fabsf(s)) + 1e-3f;
         const int ha = (int)ceilf(hs - rad), wa = (int)ceilf(ws - rad);
@@ -453,85 +490,85 @@ __global__ __launch_bounds__(64) void sm_interior_kernel(SmArgs p)
         const float yu = fmaf(s, hc, fmaf(c, wc, cx));
         const bool hit = h >= 0 && h < g.H && w >= 0 && w < g.W && xu >= 0.0f && xu <= xmax && yu >= 0.0f &&
                          yu <= ymax && (int)xu == Y && (int)yu == X;
-        const int wmin = wave_min(wa);  // the block's candidates span fewer than kSmRatRows columns
         const float fy = __builtin_amdgcn_fractf(xu), fx = __builtin_amdgcn_fractf(yu);
         const float e1 = em * (1.0f - fy), e0 = em * fy;
-        uint32_t m_ao = hit ? (uint32_t)((h * g.W + w) * g.B) * 4u : kSmOff;
-        int m_rr = hit ? min(max(w - wmin, 0), kSmRatRows - 1) : 0;
-        float m00 = hit ? e1 * (1.0f - fx) : 0.0f, m01 = hit ? e1 * fx : 0.0f;
-        float m10 = hit ? e0 * (1.0f - fx) : 0.0f, m11 = hit ? e0 * fx : 0.0f;
-        // pack each cell's hits into its first lanes: lane 4c + r takes the r-th hit
-        const uint64_t hm = __ballot(hit);
-        const uint32_t qm = (uint32_t)(hm >> (4 * cell)) & 0xFu;
-        int src = -1, cnt = 0;
-#pragma unroll
-        for (int t = 0; t < 4; ++t) {
-            const bool set = (qm >> t) & 1u;
-            src = (set && cnt == q) ? t : src;
-            cnt += set ? 1 : 0;
-        }
-        const int from = 4 * (cell + 0) + (src < 0 ? q : src);
-        m_ao = (uint32_t)__shfl((int)m_ao, from);
-        m_rr = __shfl(m_rr, from);
-        m00 = __shfl(m00, from);
-        m01 = __shfl(m01, from);
-        m10 = __shfl(m10, from);
-        m11 = __shfl(m11, from);
-        if (src < 0) {
-            m_ao = kSmOff;
-            m00 = m01 = m10 = m11 = 0.0f;
-        }
-        const int npass = wave_max(cnt);
+        m.ao = (uint32_t)((h * g.W + w) * g.B) * 4u;
+        m.ro = (uint32_t)(w * g.B) * 4u;
+        m.w00 = e1 * (1.0f - fx);
+        m.w01 = e1 * fx;
+        m.w10 = e0 * (1.0f - fx);
+        m.w11 = e0 * fx;
+        return __ballot(hit);
+    };
 
-        if constexpr (RATIO) {
-            const __amdgpu_buffer_rsrc_t rq = make_rsrc(p.ratio + (int64_t)a * WB, (int)WB);
-#pragma unroll
-            for (int r = 0; r < kSmRatRows; ++r) {
-                const int ww = wmin + r;
-                const uint32_t ro = (ww >= 0 && ww < g.W) ? voff + (uint32_t)(ww * g.B) * 4u : kSmOff;
-                rat[r * 64 + lane] = sm_load<SPL, 0>(rq, ro);
-            }
+    // The hits are streamed through a ring of RING load slots across angles:
+    // `cur` is the angle whose hits are being issued, `nxt` the next angle with
+    // hits, tested ahead so that the ring does not drain between angles.
+    SmCand cur{}, nxt{};
+    uint64_t mc = 0, mx = 0;
+    int ac = 0, ax = 0, an = 0;
+    __amdgpu_buffer_rsrc_t ra = make_rsrc(p.atten, 0), rq = ra;
+    auto find_next = [&]() {
+        while (mx == 0 && an < g.A) {
+            ax = an;
+            mx = test(an, nxt);
+            ++an;
         }
+    };
 
-        for (int ps = 0; ps < npass; ++ps) {
-            SmVec<SPL> va[RING], vr[RING];
-#pragma unroll
-            for (int i = 0; i < kSmTB * kSmTB + RING; ++i) {
-                // consume cell i - RING first: its ring slot is the one cell i's loads take
-                if (i >= RING) {
-                    const int cl = i - RING;
-                    const int L = 4 * cl + ps;
-                    SmVec<SPL> av = va[cl % RING];
-                    SmVec<SPL> rv = vr[cl % RING];
-#pragma unroll
-                    for (int j = 0; j < SPL; ++j) {
-                        // keep the product here: computed at load time it would make
-                        // every step wait for its own loads
-                        asm volatile("" : "+v"(av.v[j]), "+v"(rv.v[j]));
-                    }
-                    const float a00 = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(m00), L));
-                    const float a01 = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(m01), L));
-                    const float a10 = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(m10), L));
-                    const float a11 = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(m11), L));
-                    const int v = (cl >> 2) * kSmWB + (cl & 3);
-#pragma unroll
-                    for (int j = 0; j < SPL; ++j) {
-                        const float P = RATIO ? av.v[j] * rv.v[j] : av.v[j];
-                        acc[v].v[j] = fmaf(a00, P, acc[v].v[j]);
-                        acc[v + 1].v[j] = fmaf(a01, P, acc[v + 1].v[j]);
-                        acc[v + kSmWB].v[j] = fmaf(a10, P, acc[v + kSmWB].v[j]);
-                        acc[v + kSmWB + 1].v[j] = fmaf(a11, P, acc[v + kSmWB + 1].v[j]);
-                    }
-                }
-                if (i < kSmTB * kSmTB) {
-                    const int L = 4 * i + ps;
-                    // the point's offset in this lane's vector address (a VALU-written
-                    // SGPR read by a memory instruction would cost 5 wait states)
-                    va[i % RING] = sm_load<SPL, 2>(ra, voff + (uint32_t)__builtin_amdgcn_readlane((int)m_ao, L));
-                    if constexpr (RATIO) vr[i % RING] = rat[__builtin_amdgcn_readlane(m_rr, L) * 64 + lane];
-                }
-            }
+    SmVec<SPL> va[RING], vr[RING];
+    int sc[RING];  // cell of the hit in each slot, -1: empty
+    float s00[RING], s01[RING], s10[RING], s11[RING];
+
+    // the next hit into slot r (loads through an offset past the buffers when none is left)
+    auto issue = [&](int r) {
+        if (mc == 0 && mx != 0) {
+            cur = nxt;
+            mc = mx;
+            ac = ax;
+            mx = 0;
+            ra = make_rsrc(p.atten + (int64_t)ac * HWB, (int)HWB);
+            if constexpr (RATIO) rq = make_rsrc(p.ratio + (int64_t)ac * WB, (int)WB);
         }
+        uint32_t ao = kSmOff, ro = kSmOff;
+        sc[r] = -1;
+        if (mc != 0) {
+            const int l = __builtin_ctzll(mc);
+            mc &= mc - 1;
+            sc[r] = l >> 2;
+            s00[r] = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(cur.w00), l));
+            s01[r] = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(cur.w01), l));
+            s10[r] = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(cur.w10), l));
+            s11[r] = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(cur.w11), l));
+            // the point's offset in this lane's vector address (a VALU-written
+            // SGPR read by a memory instruction would cost 5 wait states)
+            ao = (uint32_t)__builtin_amdgcn_readlane((int)cur.ao, l);
+            ro = (uint32_t)__builtin_amdgcn_readlane((int)cur.ro, l);
+        }
+        va[r] = sm_load<SPL, 2>(ra, voff + ao);
+        if constexpr (RATIO) vr[r] = sm_load<SPL, 0>(rq, voff + ro);
+    };
+
+    find_next();
+#pragma unroll
+    for (int r = 0; r < RING; ++r) issue(r);
+    for (;;) {
+        find_next();
+        bool busy = false;
+#pragma unroll
+        for (int r = 0; r < RING; ++r) {
+            if (sc[r] >= 0) {  // wave-uniform
+                busy = true;
+                SmVec<SPL> P = va[r];
+                if constexpr (RATIO) {
+#pragma unroll
+                    for (int j = 0; j < SPL; ++j) P.v[j] = P.v[j] * vr[r].v[j];
+                }
+                sm_acc_dispatch<0, kSmTB * kSmTB - 1, SPL>(sc[r], acc, P, s00[r], s01[r], s10[r], s11[r]);
+            }
+            issue(r);
+        }
+        if (!busy) break;
     }
 
     // ---- the window: own 4 x 4 voxels, then the halo row and column ---------
