@@ -51,7 +51,6 @@ namespace fl {
 constexpr int kSmT = 8;                  // edge strips: 8 border base cells, 8-voxel depth bins
 constexpr int kSmWin = kSmT + 1;         // edge window length along the border
 constexpr int kSmEdgeVox = 2 * kSmWin;   // voxels of an edge strip's window
-constexpr int kSmPre = 8;                // edge hits whose loads are in flight together
 constexpr int kSmTB = 4;                 // interior block: 4 x 4 base cells
 constexpr int kSmWB = kSmTB + 1;         // interior window side
 constexpr int kSmWBVox = kSmWB * kSmWB;  // 25
@@ -102,19 +101,22 @@ struct SmItem {
 
 __device__ __forceinline__ SmItem sm_item(const SmGeom &g, int k, float xmax, float ymax)
 {
+    // k runs over depth bins first (the bins next to the border hold most hits,
+    // so their items start first), then strips (top, bottom, left, right), then
+    // angle ranges
     SmItem it;
     const float cx = (g.W - 1) * 0.5f, cy = (g.H - 1) * 0.5f;
     const float rlo = cy - g.reach, rhi = cy + g.reach, clo = cx - g.reach, chi = cx + g.reach;
+    const int nstrips = 2 * g.nsj + 2 * g.nsi;
     it.ch = k % g.nch;
     k /= g.nch;
-    const int ntb = g.nsj * g.nd;
-    if (k < 2 * ntb) {
-        const bool top = k < ntb;
-        if (!top) k -= ntb;
+    int st = k % nstrips;
+    it.d = k / nstrips;
+    const bool last_d = it.d == g.nd - 1;
+    if (st < 2 * g.nsj) {
+        const bool top = st < g.nsj;
         it.side = top ? kSmTop : kSmBottom;
-        it.t = k / g.nd;
-        it.d = k % g.nd;
-        const bool last_d = it.d == g.nd - 1;
+        it.t = top ? st : st - g.nsj;
         if (top) {
             it.r0 = last_d ? rlo : -(float)(8 * (it.d + 1));
             it.r1 = -(float)(8 * it.d);
@@ -129,14 +131,10 @@ __device__ __forceinline__ SmItem sm_item(const SmGeom &g, int k, float xmax, fl
         it.wx = 8 * it.t;
         return it;
     }
-    k -= 2 * ntb;
-    const int nlr = g.nsi * g.nd;
-    const bool left = k < nlr;
-    if (!left) k -= nlr;
+    st -= 2 * g.nsj;
+    const bool left = st < g.nsi;
     it.side = left ? kSmLeft : kSmRight;
-    it.t = k / g.nd;
-    it.d = k % g.nd;
-    const bool last_d = it.d == g.nd - 1;
+    it.t = left ? st : st - g.nsi;
     it.r0 = (float)(8 * it.t);
     it.r1 = (float)(8 * it.t + 8);
     if (left) {
@@ -181,26 +179,67 @@ __device__ __forceinline__ SmVec<N> sm_load(__amdgpu_buffer_rsrc_t r, uint32_t o
 }
 
 // ---------------------------------------------------------------------------
-// Edge strips: the hits' base voxels vary per hit, so the window is an LDS array
-// (2 x 9 voxels for the top and bottom strips, 9 x 2 for the left and right ones,
-// pitch `pw`), 64 * SPL slices per voxel, read-add-written per hit.  The kSmEW
-// wavefronts of a workgroup share one item and take its angles in turn, each into
-// its own window; the windows are summed in wavefront order at the end.
+// Edge strips.  In strip coordinates a voxel of the window is (across, along):
+// across 0/1 is the window's first/second row (top and bottom strips) or
+// column (left and right strips), along 0..8 runs with the border.  A hit whose
+// base voxel is at along = P adds to (0, P), (0, P + 1), (1, P), (1, P + 1); the
+// window is 18 * SPL registers per lane, and each hit is dispatched to the
+// registers of its P through a tree of wave-uniform branches.  The kSmEW
+// wavefronts of a workgroup share one item and take its angles in turn; their
+// windows are summed in wavefront order through LDS at the end.
 // ---------------------------------------------------------------------------
 constexpr int kSmEW = 4;  // wavefronts per edge item
+
+template <int P, int SPL>
+__device__ __forceinline__ void sm_edge_acc(SmVec<SPL> *acc, const SmVec<SPL> &v, float a00, float a01, float a10,
+                                            float a11)
+{
+#pragma unroll
+    for (int j = 0; j < SPL; ++j) {
+        acc[P].v[j] = fmaf(a00, v.v[j], acc[P].v[j]);
+        acc[P + 1].v[j] = fmaf(a01, v.v[j], acc[P + 1].v[j]);
+        acc[kSmWin + P].v[j] = fmaf(a10, v.v[j], acc[kSmWin + P].v[j]);
+        acc[kSmWin + P + 1].v[j] = fmaf(a11, v.v[j], acc[kSmWin + P + 1].v[j]);
+    }
+    // a marker that differs per position (see sm_acc_cell)
+    asm volatile("; strip position %0" ::"n"(P));
+}
+
+template <int LO, int HI, int SPL>
+__device__ __forceinline__ void sm_edge_dispatch(int pos, SmVec<SPL> *acc, const SmVec<SPL> &v, float a00, float a01,
+                                                 float a10, float a11)
+{
+    if constexpr (LO == HI) {
+        sm_edge_acc<LO, SPL>(acc, v, a00, a01, a10, a11);
+    } else {
+        constexpr int MID = (LO + HI) / 2;
+        if (pos <= MID)
+            sm_edge_dispatch<LO, MID, SPL>(pos, acc, v, a00, a01, a10, a11);
+        else
+            sm_edge_dispatch<MID + 1, HI, SPL>(pos, acc, v, a00, a01, a10, a11);
+    }
+}
+
+// Per-lane fields of one round of candidates: offsets, position along the strip
+// and the four weights in strip coordinates (the emission factor folded in).
+struct SmEdgeCand {
+    uint32_t ao, ro;
+    int pos;
+    float w00, w01, w10, w11;
+};
 
 template <int SPL, bool RATIO>
 __global__ __launch_bounds__(64 * kSmEW) void sm_edge_kernel(SmArgs p)
 {
     constexpr int S = 64 * SPL;
-    __shared__ SmVec<SPL> win_all[kSmEW * kSmEdgeVox * 64];
+    constexpr int RING = 8;
+    __shared__ SmVec<SPL> win[kSmEdgeVox * 64];
     const SmGeom &g = p.g;
-    const int per_group = sm_edge_items(g);
-    const uint32_t id = xcd_remap(blockIdx.x, gridDim.x);
-    const int grp = (int)(id / (uint32_t)per_group);
-    const int k = (int)(id % (uint32_t)per_group);
+    // items in dispatch order (the dispatcher deals consecutive workgroups to
+    // the XCDs in turn), slice groups innermost
+    const int grp = (int)(blockIdx.x % (uint32_t)g.Gi);
+    const int k = (int)(blockIdx.x / (uint32_t)g.Gi);
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    SmVec<SPL> *win = win_all + wave * kSmEdgeVox * 64;
     const int b0 = grp * S + lane * SPL;  // this lane's first slice (B is a multiple of SPL)
     const float xmax = g.H - 1.001f, ymax = g.W - 1.001f;
     const float cx = (g.W - 1) * 0.5f, cy = (g.H - 1) * 0.5f;
@@ -210,12 +249,13 @@ __global__ __launch_bounds__(64 * kSmEW) void sm_edge_kernel(SmArgs p)
     // lanes past the last slice read through an offset past every buffer (0)
     const uint32_t voff = b0 < g.B ? (uint32_t)b0 * 4u : kSmOff;
     const bool rows = it.side == kSmTop || it.side == kSmBottom;
-    const int pw = rows ? kSmWin : 2;  // window pitch (voxels)
     const int a_begin = (int)((int64_t)g.A * it.ch / g.nch), a_end = (int)((int64_t)g.A * (it.ch + 1) / g.nch);
 
+    SmVec<SPL> acc[kSmEdgeVox];
+#pragma unroll
     for (int v = 0; v < kSmEdgeVox; ++v)
 #pragma unroll
-        for (int j = 0; j < SPL; ++j) win[v * 64 + lane].v[j] = 0.0f;
+        for (int j = 0; j < SPL; ++j) acc[v].v[j] = 0.0f;
 
     // Every sample lies in the disc of radius `reach` about the centre (the
     // rotation preserves distances): clip the region's columns to the disc's
@@ -232,141 +272,174 @@ __global__ __launch_bounds__(64 * kSmEW) void sm_edge_kernel(SmArgs p)
     }
     const bool reachable = r0 <= r1 && c0 <= c1;
 
-    for (int a = a_begin + wave; a < a_end && reachable; a += kSmEW) {
-        const float th = finite_or_zero(scalar_read(p.theta, a));
-        const float c = cosf(-th), s = sinf(-th);
-        // bounding box of the preimage of the item's region (inverse rotation of
-        // its corners), one lattice step of margin, clipped to the detector
+    // ---- the test cursor: this wavefront's angles a_begin + wave + n * kSmEW,
+    // each enumerating the lattice points of the bounding box of the preimage of
+    // the item's region (inverse rotation of its corners, one lattice step of
+    // margin, clipped to the detector), 64 per round
+    int ta = reachable ? a_begin + wave : a_end;  // angle under test
+    int tb = 0, ncand = 0;                        // next candidate of that angle, candidates
+    int h0 = 0, w0 = 0, nw = 1;
+    float tc = 1.0f, ts = 0.0f, tem = 0.0f, inv_nw = 1.0f;
+    auto open_angle = [&]() {  // bounding box of angle ta; ncand = 0 when empty
+        const float th = finite_or_zero(scalar_read(p.theta, ta));
+        tc = cosf(-th);
+        ts = sinf(-th);
+        tem = scalar_read(p.em, ta);
         float hlo = 3.0e38f, hhi = -3.0e38f, wlo = 3.0e38f, whi = -3.0e38f;
 #pragma unroll
-        for (int q = 0; q < 4; ++q) {
-            const float rr = ((q & 1) ? r1 : r0) - cy;
-            const float cc = ((q & 2) ? c1 : c0) - cx;
-            const float hh = c * rr + s * cc, ww = -s * rr + c * cc;
+        for (int qq = 0; qq < 4; ++qq) {
+            const float rr = ((qq & 1) ? r1 : r0) - cy;
+            const float cc = ((qq & 2) ? c1 : c0) - cx;
+            const float hh = tc * rr + ts * cc, ww = -ts * rr + tc * cc;
             hlo = fminf(hlo, hh);
             hhi = fmaxf(hhi, hh);
             wlo = fminf(wlo, ww);
             whi = fmaxf(whi, ww);
         }
-        const int h0 = max(0, (int)floorf(hlo + cy) - 1), h1 = min(g.H - 1, (int)ceilf(hhi + cy) + 1);
-        const int w0 = max(0, (int)floorf(wlo + cx) - 1), w1 = min(g.W - 1, (int)ceilf(whi + cx) + 1);
-        if (h0 > h1 || w0 > w1) continue;
-        const int nw = w1 - w0 + 1;
-        const int ncand = (h1 - h0 + 1) * nw;
-        const float inv_nw = 1.0f / (float)nw;
-        const float em = scalar_read(p.em, a);
-        const __amdgpu_buffer_rsrc_t ra = make_rsrc(p.atten + (int64_t)a * HWB, (int)HWB);
-        __amdgpu_buffer_rsrc_t rq = ra;
-        if constexpr (RATIO) rq = make_rsrc(p.ratio + (int64_t)a * WB, (int)WB);
+        h0 = max(0, (int)floorf(hlo + cy) - 1);
+        const int h1 = min(g.H - 1, (int)ceilf(hhi + cy) + 1);
+        w0 = max(0, (int)floorf(wlo + cx) - 1);
+        const int w1 = min(g.W - 1, (int)ceilf(whi + cx) + 1);
+        nw = max(w1 - w0 + 1, 1);
+        ncand = (h0 > h1 || w0 > w1) ? 0 : (h1 - h0 + 1) * nw;
+        inv_nw = 1.0f / (float)nw;
+        tb = 0;
+    };
+    // one round of candidates: lane = one lattice point; returns the hit mask
+    auto test = [&](SmEdgeCand &m) -> uint64_t {
+        const int kc = tb + lane;
+        int qh = (int)(((float)kc + 0.5f) * inv_nw);
+        int qw = kc - qh * nw;
+        if (qw < 0) {
+            --qh;
+            qw += nw;
+        } else if (qw >= nw) {
+            ++qh;
+            qw -= nw;
+        }
+        const int h = h0 + qh, w = w0 + qw;
+        bool hit = kc < ncand;
+        // the forward projector's position arithmetic, unclamped
+        const float hc = (float)h - cy, wc = (float)w - cx;
+        const float xu = fmaf(tc, hc, fmaf(-ts, wc, cy));  // row
+        const float yu = fmaf(ts, hc, fmaf(tc, wc, cx));   // col
+        const bool rin = xu >= 0.0f && xu <= xmax;
+        const float xc = clampf(xu, 0.0f, xmax), yc = clampf(yu, 0.0f, ymax);
+        const int y0 = (int)xc, x0 = (int)yc;
+        const float fy = __builtin_amdgcn_fractf(xc), fx = __builtin_amdgcn_fractf(yc);
+        if (it.side == kSmTop) {
+            hit = hit && xu < 0.0f && (x0 >> 3) == it.t && sm_depth(-xu, g.nd) == it.d;
+        } else if (it.side == kSmBottom) {
+            hit = hit && xu > xmax && (x0 >> 3) == it.t && sm_depth(xu - xmax, g.nd) == it.d;
+        } else if (it.side == kSmLeft) {
+            hit = hit && rin && yu < 0.0f && (y0 >> 3) == it.t && sm_depth(-yu, g.nd) == it.d;
+        } else {
+            hit = hit && rin && yu > ymax && (y0 >> 3) == it.t && sm_depth(yu - ymax, g.nd) == it.d;
+        }
+        m.ao = (uint32_t)((h * g.W + w) * g.B) * 4u;
+        m.ro = (uint32_t)(w * g.B) * 4u;
+        m.pos = rows ? x0 - it.wx : y0 - it.wy;
+        const float e1 = tem * (1.0f - fy), e0 = tem * fy;
+        // image weights: (y0, x0), (y0, x0 + 1), (y0 + 1, x0), (y0 + 1, x0 + 1);
+        // a column strip runs along rows, so its (0, P + 1) is (y0 + 1, x0)
+        const float i01 = e1 * fx, i10 = e0 * (1.0f - fx);
+        m.w00 = e1 * (1.0f - fx);
+        m.w01 = rows ? i01 : i10;
+        m.w10 = rows ? i10 : i01;
+        m.w11 = e0 * fx;
+        tb += 64;
+        return __ballot(hit);
+    };
 
-        for (int base = 0; base < ncand; base += 64) {
-            // ---- candidate test: lane = one lattice point of the box ----------
-            const int kc = base + lane;
-            int qh = (int)(((float)kc + 0.5f) * inv_nw);
-            int qw = kc - qh * nw;
-            if (qw < 0) {
-                --qh;
-                qw += nw;
-            } else if (qw >= nw) {
-                ++qh;
-                qw -= nw;
+    // ---- hits streamed through a ring of RING load slots ---------------------
+    SmEdgeCand cur{}, nxt{};
+    uint64_t mc = 0, mx = 0;
+    int ac = 0, ax = 0;
+    __amdgpu_buffer_rsrc_t ra = make_rsrc(p.atten, 0), rq = ra;
+    auto find_next = [&]() {
+        while (mx == 0 && ta < a_end) {
+            if (tb == 0) open_angle();
+            if (tb < ncand) {
+                ax = ta;
+                mx = test(nxt);
             }
-            const int h = h0 + qh, w = w0 + qw;
-            bool hit = kc < ncand;
-            // the forward projector's position arithmetic, unclamped
-            const float hc = (float)h - cy, wc = (float)w - cx;
-            const float xu = fmaf(c, hc, fmaf(-s, wc, cy));  // row
-            const float yu = fmaf(s, hc, fmaf(c, wc, cx));   // col
-            const bool rin = xu >= 0.0f && xu <= xmax;
-            const float xc = clampf(xu, 0.0f, xmax), yc = clampf(yu, 0.0f, ymax);
-            const int y0 = (int)xc, x0 = (int)yc;
-            const float fy = __builtin_amdgcn_fractf(xc), fx = __builtin_amdgcn_fractf(yc);
-            if (it.side == kSmTop) {
-                hit = hit && xu < 0.0f && (x0 >> 3) == it.t && sm_depth(-xu, g.nd) == it.d;
-            } else if (it.side == kSmBottom) {
-                hit = hit && xu > xmax && (x0 >> 3) == it.t && sm_depth(xu - xmax, g.nd) == it.d;
-            } else if (it.side == kSmLeft) {
-                hit = hit && rin && yu < 0.0f && (y0 >> 3) == it.t && sm_depth(-yu, g.nd) == it.d;
-            } else {
-                hit = hit && rin && yu > ymax && (y0 >> 3) == it.t && sm_depth(yu - ymax, g.nd) == it.d;
-            }
-            // per-hit fields: byte offsets of the point's slices and ratio, the
-            // window word of its base voxel, its four bilinear weights (lanes
-            // without a hit: an offset past the buffers and weights 0)
-            const uint32_t aoff = hit ? (uint32_t)((h * g.W + w) * g.B) * 4u : kSmOff;
-            const uint32_t roff = hit ? (uint32_t)(w * g.B) * 4u : kSmOff;
-            const int wv = hit ? (y0 - it.wy) * pw + (x0 - it.wx) : 0;
-            const float w00 = (1.0f - fy) * (1.0f - fx), w01 = (1.0f - fy) * fx;
-            const float w10 = fy * (1.0f - fx), w11 = fy * fx;
-            uint64_t mask = __ballot(hit);
-
-            // ---- walk the hits, kSmPre at a time.  The loads are unconditional
-            // (an empty slot reads 0 through an offset past the buffers), so the
-            // later ones stay in flight while the earlier hits are added
-            while (mask) {
-                int ln[kSmPre];
-                int nh = 0;
-#pragma unroll
-                for (int i = 0; i < kSmPre; ++i) {
-                    ln[i] = mask ? __builtin_ctzll(mask) : 64;
-                    nh += mask ? 1 : 0;
-                    mask &= mask - 1;
-                }
-                SmVec<SPL> val[kSmPre], rv[kSmPre];
-#pragma unroll
-                for (int i = 0; i < kSmPre; ++i) {
-                    const uint32_t ao = i < nh ? (uint32_t)__builtin_amdgcn_readlane((int)aoff, ln[i] & 63) : kSmOff;
-                    val[i] = sm_load<SPL, 2>(ra, voff + ao);
-                    if constexpr (RATIO) {
-                        const uint32_t ro = i < nh ? (uint32_t)__builtin_amdgcn_readlane((int)roff, ln[i] & 63) : kSmOff;
-                        rv[i] = sm_load<SPL, 0>(rq, voff + ro);
-                    }
-                }
-#pragma unroll
-                for (int i = 0; i < kSmPre; ++i) {
-                    if (i < nh) {  // wave-uniform
-                        const int l = ln[i];
-                        const int vv = __builtin_amdgcn_readlane(wv, l);
-                        const float a00 = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(w00), l));
-                        const float a01 = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(w01), l));
-                        const float a10 = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(w10), l));
-                        const float a11 = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(w11), l));
-                        SmVec<SPL> *o = win + vv * 64 + lane;
-                        SmVec<SPL> o00 = o[0], o01 = o[64], o10 = o[pw * 64], o11 = o[(pw + 1) * 64];
-#pragma unroll
-                        for (int j = 0; j < SPL; ++j) {
-                            float P = em * val[i].v[j];
-                            if constexpr (RATIO) P = em * (val[i].v[j] * rv[i].v[j]);
-                            o00.v[j] = fmaf(a00, P, o00.v[j]);
-                            o01.v[j] = fmaf(a01, P, o01.v[j]);
-                            o10.v[j] = fmaf(a10, P, o10.v[j]);
-                            o11.v[j] = fmaf(a11, P, o11.v[j]);
-                        }
-                        o[0] = o00;
-                        o[64] = o01;
-                        o[pw * 64] = o10;
-                        o[(pw + 1) * 64] = o11;
-                    }
-                }
+            if (tb >= ncand) {
+                ta += kSmEW;
+                tb = 0;
             }
         }
+    };
+
+    SmVec<SPL> va[RING], vr[RING];
+    int sp[RING];  // position of the hit in each slot, -1: empty
+    float s00[RING], s01[RING], s10[RING], s11[RING];
+    auto issue = [&](int r) {
+        if (mc == 0 && mx != 0) {
+            cur = nxt;
+            mc = mx;
+            ac = ax;
+            mx = 0;
+            ra = make_rsrc(p.atten + (int64_t)ac * HWB, (int)HWB);
+            if constexpr (RATIO) rq = make_rsrc(p.ratio + (int64_t)ac * WB, (int)WB);
+        }
+        uint32_t ao = kSmOff, ro = kSmOff;
+        sp[r] = -1;
+        if (mc != 0) {
+            const int l = __builtin_ctzll(mc);
+            mc &= mc - 1;
+            sp[r] = __builtin_amdgcn_readlane(cur.pos, l);
+            s00[r] = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(cur.w00), l));
+            s01[r] = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(cur.w01), l));
+            s10[r] = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(cur.w10), l));
+            s11[r] = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(cur.w11), l));
+            ao = (uint32_t)__builtin_amdgcn_readlane((int)cur.ao, l);
+            ro = (uint32_t)__builtin_amdgcn_readlane((int)cur.ro, l);
+        }
+        va[r] = sm_load<SPL, 2>(ra, voff + ao);
+        if constexpr (RATIO) vr[r] = sm_load<SPL, 0>(rq, voff + ro);
+    };
+
+    find_next();
+#pragma unroll
+    for (int r = 0; r < RING; ++r) issue(r);
+    for (;;) {
+        find_next();
+        bool busy = false;
+#pragma unroll
+        for (int r = 0; r < RING; ++r) {
+            if (sp[r] >= 0) {  // wave-uniform
+                busy = true;
+                SmVec<SPL> v = va[r];
+                if constexpr (RATIO) {
+#pragma unroll
+                    for (int j = 0; j < SPL; ++j) v.v[j] = v.v[j] * vr[r].v[j];
+                }
+                sm_edge_dispatch<0, kSmT - 1, SPL>(sp[r], acc, v, s00[r], s01[r], s10[r], s11[r]);
+            }
+            issue(r);
+        }
+        if (!busy) break;
     }
 
-    // ---- the windows summed in wavefront order; entry q = across * 9 + along -
-    __syncthreads();
+    // ---- the wavefronts' windows summed in wavefront order; entry q = across * 9 + along
+    for (int u = 0; u < kSmEW; ++u) {
+        if (wave == u) {
+#pragma unroll
+            for (int v = 0; v < kSmEdgeVox; ++v) {
+                SmVec<SPL> x = acc[v];
+                if (u > 0) {
+                    const SmVec<SPL> y = win[v * 64 + lane];
+#pragma unroll
+                    for (int j = 0; j < SPL; ++j) x.v[j] = y.v[j] + x.v[j];
+                }
+                win[v * 64 + lane] = x;
+            }
+        }
+        __syncthreads();
+    }
     SmVec<SPL> *e = reinterpret_cast<SmVec<SPL> *>(p.edge + sm_edge_index(g, grp, it.side, it.t, it.d, it.ch) *
                                                                 kSmEdgeVox * S);
-    for (int q = wave; q < kSmEdgeVox; q += kSmEW) {
-        const int across = q / kSmWin, along = q % kSmWin;
-        const int v = rows ? across * kSmWin + along : along * 2 + across;
-        SmVec<SPL> x = win_all[v * 64 + lane];
-        for (int u = 1; u < kSmEW; ++u) {
-            const SmVec<SPL> y = win_all[(u * kSmEdgeVox + v) * 64 + lane];
-#pragma unroll
-            for (int j = 0; j < SPL; ++j) x.v[j] += y.v[j];
-        }
-        e[q * 64 + lane] = x;
-    }
+    for (int q = wave; q < kSmEdgeVox; q += kSmEW) e[q * 64 + lane] = win[q * 64 + lane];
 }
 
 // Sum of a strip's windows over its depth bins and angle ranges, in that fixed
