@@ -66,6 +66,10 @@ SIGNATURES = {
     "fl_mlem_multi_create": (_INT, [_P, _I64, _D, _D, _P, _SZ, _P, ctypes.POINTER(_P)]),
     "fl_mlem_multi_run": (_INT, [_P, _I64, _INT, _P]),
     "fl_mlem_multi_destroy": (_INT, [_P]),
+    "fl_frac_sum": (_INT, [_P, _P, _I64, _I64, _INT, _P]),
+    "fl_adaptive_mask": (_INT, [_P, _P, _I64, _I64, _I64, _INT, _INT, _INT, _INT, _P, _P, _P]),
+    "fl_frac_ratio": (_INT, [_P, _P, _P, _I64, _I64, _INT, _INT, _P]),
+    "fl_frac_final": (_INT, [_P, _P, _P, _P, _P, _I64, _I64, _INT, _P]),
 }
 
 

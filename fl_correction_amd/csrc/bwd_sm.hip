@@ -447,7 +447,6 @@ __global__ __launch_bounds__(64 * kSmEW) void sm_edge_kernel(SmArgs p)
 template <int SPL>
 __global__ __launch_bounds__(64) void sm_edge_reduce_kernel(SmArgs p)
 {
-    constexpr int S = 64 * SPL;
     const SmGeom &g = p.g;
     const int ns = g.nsi > g.nsj ? g.nsi : g.nsj;
     int id = blockIdx.x;

@@ -42,6 +42,13 @@ int launch_combine(const float *T_rot, const float *incident, float *out, int64_
                    int64_t K, int64_t B, int64_t H, int64_t W, int side, hipStream_t s);
 int launch_cone_direction(const float *mu, float *T, int64_t K, int64_t Z, int64_t X, int64_t Y, int pz, int py,
                           int q, float weight, int accumulate, int reverse, hipStream_t s);
+int launch_frac_sum(const void *img, void *out, int64_t n, int64_t npix, int f64, hipStream_t s);
+int launch_adaptive_mask(const void *img, double *mask, int64_t n_img, int64_t H, int64_t W, int fill_hole,
+                         int dilation, int erosion, int f64, int *status, void *thresholds, hipStream_t s);
+int launch_frac_ratio(const void *img, const double *img_sum, double *tmp, int64_t n, int64_t npix, int f64,
+                      int single, hipStream_t s);
+int launch_frac_final(const double *tmp, const double *mask, const double *ratio, const double *lohi, double *frac,
+                      int64_t n, int64_t npix, int single, hipStream_t s);
 
 static int check_dims(int64_t A, int64_t B, int64_t R, int64_t H, int64_t W)
 {
@@ -554,6 +561,47 @@ int fl_xrf_combine(const float *T_rot, const float *incident, float *out, int64_
     FL_REQUIRE(T_rot && out, "null pointer argument");
     return launch_combine(T_rot, incident, out, inc_element_stride, out_element_stride, K, B, H, W, side,
                           (hipStream_t)stream);
+}
+
+int fl_frac_sum(const void *img, void *out, int64_t n, int64_t npix, int f64, void *stream)
+{
+    g_last_error.clear();
+    FL_REQUIRE(n >= 1 && npix >= 0, "shape: need n >= 1, npix >= 0");
+    if (npix == 0) return FL_OK;
+    FL_REQUIRE(img && out, "null pointer argument");
+    return launch_frac_sum(img, out, n, npix, f64 != 0, (hipStream_t)stream);
+}
+
+int fl_adaptive_mask(const void *img, double *mask, int64_t n_img, int64_t H, int64_t W, int fill_hole,
+                     int dilation, int erosion, int f64, int *status, void *thresholds, void *stream)
+{
+    g_last_error.clear();
+    FL_REQUIRE(n_img >= 0 && H >= 1 && W >= 1 && H < (1 << 20) && W < (1 << 20), "shape: bad size");
+    FL_REQUIRE(dilation >= 0 && erosion >= 0, "dilation and erosion must be >= 0");
+    if (n_img == 0) return FL_OK;
+    FL_REQUIRE(img && mask && status && thresholds, "null pointer argument");
+    return launch_adaptive_mask(img, mask, n_img, H, W, fill_hole != 0, dilation, erosion, f64 != 0, status,
+                                thresholds, (hipStream_t)stream);
+}
+
+int fl_frac_ratio(const void *img, const double *img_sum, double *tmp, int64_t n, int64_t npix, int f64,
+                  int single, void *stream)
+{
+    g_last_error.clear();
+    FL_REQUIRE(n >= 0 && npix >= 0, "shape: bad size");
+    if (n == 0 || npix == 0) return FL_OK;
+    FL_REQUIRE(img && img_sum && tmp, "null pointer argument");
+    return launch_frac_ratio(img, img_sum, tmp, n, npix, f64 != 0, single != 0, (hipStream_t)stream);
+}
+
+int fl_frac_final(const double *tmp, const double *mask, const double *ratio, const double *lohi, double *frac,
+                  int64_t n, int64_t npix, int single, void *stream)
+{
+    g_last_error.clear();
+    FL_REQUIRE(n >= 0 && npix >= 0, "shape: bad size");
+    if (n == 0 || npix == 0) return FL_OK;
+    FL_REQUIRE(tmp && lohi && frac, "null pointer argument");
+    return launch_frac_final(tmp, mask, ratio, lohi, frac, n, npix, single != 0, (hipStream_t)stream);
 }
 
 }  // extern "C"

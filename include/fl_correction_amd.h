@@ -115,7 +115,7 @@ FL_API int fl_mlem_batch_noalloc(const float *C_init, const float *atten, const 
 /* ---------------------------------------------------------------------------
  * Slice-major path.  The back-projection reads the attenuation as a slice-major
  * copy atten_sm (A, H, W, B): the B slices of one detector point are contiguous,
- * so one sample position and its four weights serve 64 slices at once.  The
+ * so one sample position and its four weights serve all of them at once.  The
  * results equal the reference-layout back-projector's to rounding (the sums run
  * in another fixed order).  One reference per line (R = 1; em_cs (A)); ratio_sm
  * is the MLEM ratio laid out (A, W, B); H*W*B < 2^29.  The back-projector needs
@@ -342,6 +342,40 @@ FL_API int fl_mu_adjoint(const float *dmu_rot, const float *dmux, const float *c
 /* Transpose of fl_rotate: dC (+)= Rot^T(dF) over N planes, by gather (no atomics). */
 FL_API int fl_rotate_adjoint(const float *dF, float *dC, int64_t N, int64_t H, int64_t W,
                              float cos_t, float sin_t, int accumulate, void *stream);
+
+/* ---------------------------------------------------------------------------
+ * cal_frac (FL/core.py:159-211) and its adaptive threshold (FL/image_util.py:337-375).
+ * `f64` selects the element type of img/out/thresholds: 0 float32, 1 float64.
+ * ------------------------------------------------------------------------- */
+
+/* out[p] = sum_i img[i, p] (in order, in the element type) + 1e-12; img (n, npix)
+ * (np.sum(img, axis=0) + 1e-12, core.py:171). */
+FL_API int fl_frac_sum(const void *img, void *out, int64_t n, int64_t npix, int f64, void *stream);
+
+/* adaptive_threshold (image_util.py:337-375) of n_img images (H, W) each: mask =
+ * img >= the lowest three-class multi-Otsu threshold (skimage threshold_multiotsu,
+ * 256 bins), then binary_fill_holes with a 5x5 structure if fill_hole, then
+ * `dilation` dilations and `erosion` erosions with the 4-neighbour cross.  mask
+ * (n_img, H, W) float64 0/1.  status[i] = 1 where the threshold fails (fewer than
+ * three distinct histogram bins, or a non-finite value): that image's mask is 0, as
+ * the reference's except branch sets it; else 0 and thresholds[i] holds the
+ * threshold.  H * ceil(W/32) <= 18432. */
+FL_API int fl_adaptive_mask(const void *img, double *mask, int64_t n_img, int64_t H, int64_t W,
+                            int fill_hole, int dilation, int erosion, int f64, int *status,
+                            void *thresholds, void *stream);
+
+/* tmp[i, p] = rm_abnormal(img[i, p] / img_sum[p]) in float64 (NaN, inf, < 0 -> 0);
+ * core.py:194 with image_util.py:396-401.  single != 0 rounds the quotient to float32
+ * (the reference's float32 arithmetic when its mask falls back to 1). */
+FL_API int fl_frac_ratio(const void *img, const double *img_sum, double *tmp, int64_t n,
+                         int64_t npix, int f64, int single, void *stream);
+
+/* frac[i, p] = (clip_i(tmp[i, p]) * mask[p]) * ratio[p] (core.py:195-201): clip_i to
+ * [lohi[2i], lohi[2i+1]] when lohi[2i] <= lohi[2i+1], else no clipping; mask or ratio
+ * NULL means 1; single != 0 rounds each product to float32. */
+FL_API int fl_frac_final(const double *tmp, const double *mask, const double *ratio,
+                         const double *lohi, double *frac, int64_t n, int64_t npix, int single,
+                         void *stream);
 
 #ifdef __cplusplus
 }
