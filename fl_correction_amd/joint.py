@@ -56,13 +56,27 @@ def mlem_start(model: XRFModel, I, C_init, n_iter=10, beta=0.0, delta=0.01):
     return out
 
 
+def element_fractions(C, scale_range=(0.02, 0.98), enable_scale=True, scale_limit=0.95):
+    """cal_frac's 'frac' of the element maps C (K, B, H, W) as float32 (K, B, H, W):
+    the maps the reference demo recomputes the attenuation from
+    (FL/example/demo.py:255-257, FL/core.py:159-211)."""
+    from .frac import cal_frac
+    res = cal_frac(C.contiguous(), scale_range=list(scale_range), enable_scale=enable_scale, scale_limit=scale_limit)
+    return res["frac"].to(torch.float32).contiguous()
+
+
 def self_consistent_mlem(model: XRFModel, I, C_init, n_iter=50, recompute_every=5, beta=0.0, delta=0.01,
-                         callback=None, start_unattenuated=False):
+                         callback=None, start_unattenuated=False, atten_from=None):
     """n_iter MLEM iterations per emission line, the attenuation recomputed from
     the current estimate before every block of `recompute_every` iterations.
 
     I (K, A, B, W), C_init (K, B, H, W); returns a new (K, B, H, W) tensor.
     callback(iteration, C) is called after every block if given.
+
+    atten_from: a function of the estimate C (K, B, H, W) giving the maps the
+    attenuation is recomputed from (None: C itself).  ``element_fractions``
+    recomputes it from cal_frac's masked, rescaled element fractions, as the
+    reference demo's outer loop does (FL/example/demo.py:255-257).
 
     start_unattenuated: the first block runs without attenuation (atten = 1),
     as the reference's demo makes its first reconstruction (FL/example/demo.py:
@@ -82,7 +96,8 @@ def self_consistent_mlem(model: XRFModel, I, C_init, n_iter=50, recompute_every=
             ones = torch.ones((model.A, *C.shape[1:]), dtype=torch.float32, device=model.device)
             atten = [ones] * model.K
         else:
-            atten = model.attenuation(C)  # (K, A, B, H, W) for the current estimate
+            src = C if atten_from is None else atten_from(C)
+            atten = model.attenuation(src)  # (K, A, B, H, W) for the current estimate
         for e in range(model.K):
             em = model.em_T[e][:, None].contiguous()
             solver = MLEMSolver(atten[e], em, model.theta, I[e].contiguous(), beta=beta, delta=delta)

@@ -108,3 +108,44 @@ def test_self_consistent_mlem_unattenuated_start():
     then = self_consistent_mlem(model, I, first, n_iter=3, recompute_every=3)
     assert rel_l2(both.cpu().numpy(), then.cpu().numpy()) <= 1e-6
 
+
+
+def test_self_consistent_mlem_from_element_fractions():
+    """The outer loop recomputing the attenuation from cal_frac's element fractions
+    (the reference demo's loop) against the oracle composition with the oracle's
+    cal_frac."""
+    from helpers import RTOL, oracle, rel_l2
+    import frac_oracle as fo
+    from fl_correction_amd import phantom
+    from fl_correction_amd.atten_util import make_param
+    from fl_correction_amd.detector_mask import generate_detector_mask
+    from fl_correction_amd.joint import element_fractions, self_consistent_mlem
+    from fl_correction_amd.model import XRFModel
+    K, B, N, A = 2, 3, 24, 6
+    rng = np.random.default_rng(20260012)
+    C_true = phantom.ellipsoid_phantom((B, N, N), K, n_ell=(3, 5), semi_axes=(3, 9), seed=20260012)
+    cs_x, cs_line = rng.uniform(20, 400, K), rng.uniform(20, 400, (K, K))
+    param = make_param(["Fe", "Cu"], ["Fe", "Cu"], cs_x, cs_line, rho=1.0, pix=1e-4)
+    mask = generate_detector_mask(20, 20, N).astype(np.float32)
+    angles = phantom.angle_list(A)
+    model = XRFModel(param, angles, mask, store=False)
+    I = model.forward(torch.from_numpy(C_true).cuda())
+    C0 = (C_true + 0.05 * rng.random(C_true.shape)).astype(np.float32)
+    print(f"self-consistent MLEM from element fractions: K={K} B={B} N={N} A={A}", flush=True)
+    got = self_consistent_mlem(model, I, torch.from_numpy(C0).cuda(), n_iter=4, recompute_every=2,
+                               atten_from=element_fractions).cpu().numpy()
+    I_np, em, theta = I.cpu().numpy(), model.em_T.cpu().numpy(), model.theta.cpu().numpy()
+    C = C0.copy()
+    for _ in range(2):
+        F = fo.cal_frac(C)["frac"].astype(np.float32)
+        atten = np.empty((K, A, B, N, N), np.float32)
+        for a, ang in enumerate(angles):
+            rot = oracle.rotate_torch(F, np.float32(ang / 180 * np.pi))
+            a_x = oracle.incident_f32(oracle.mu_xray(rot, (cs_x * 1e-4).astype(np.float32)))
+            a_f = oracle.xrf_atten_from_mask(oracle.mu_xrf(rot, (cs_line * 1e-4).astype(np.float32)), mask)
+            atten[:, a] = a_f * a_x
+        for e in range(K):
+            C[e] = oracle.mlem_batch_f32(C[e:e + 1], atten[e], em[e][:, None].copy(), theta, I_np[e], 2, 0.0, 0.01)[0]
+    err = rel_l2(got, C)
+    print(f"  vs oracle composition: rel-L2 {err:.2e}")
+    assert err <= RTOL
