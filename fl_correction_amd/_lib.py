@@ -66,6 +66,8 @@ SIGNATURES = {
     "fl_mlem_multi_create": (_INT, [_P, _I64, _D, _D, _P, _SZ, _P, ctypes.POINTER(_P)]),
     "fl_mlem_multi_run": (_INT, [_P, _I64, _INT, _P]),
     "fl_mlem_multi_destroy": (_INT, [_P]),
+    "fl_cone_sweep_workspace_bytes": (_SZ, [_I64, _I64, _I64, _I64, _P, _P, _P, _I64]),
+    "fl_cone_sweep": (_INT, [_P, _P, _I64, _I64, _I64, _I64, _P, _P, _P, _P, _I64, _INT, _P, _SZ, _P]),
     "fl_frac_sum": (_INT, [_P, _P, _I64, _I64, _INT, _P]),
     "fl_adaptive_mask": (_INT, [_P, _P, _I64, _I64, _I64, _INT, _INT, _INT, _INT, _P, _P, _P]),
     "fl_frac_ratio": (_INT, [_P, _P, _P, _I64, _I64, _INT, _INT, _P]),

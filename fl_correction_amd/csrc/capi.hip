@@ -42,6 +42,11 @@ int launch_combine(const float *T_rot, const float *incident, float *out, int64_
                    int64_t K, int64_t B, int64_t H, int64_t W, int side, hipStream_t s);
 int launch_cone_direction(const float *mu, float *T, int64_t K, int64_t Z, int64_t X, int64_t Y, int pz, int py,
                           int q, float weight, int accumulate, int reverse, hipStream_t s);
+size_t cone_sweep_workspace_bytes(int64_t K, int64_t Z, int64_t X, int64_t Y, const int32_t *pz, const int32_t *py,
+                                  const int32_t *q, int64_t ndir);
+int launch_cone_sweep(const float *mu, float *T, int64_t K, int64_t Z, int64_t X, int64_t Y, const int32_t *pz,
+                      const int32_t *py, const int32_t *q, const float *weight, int64_t ndir, int reverse,
+                      void *workspace, size_t workspace_bytes, hipStream_t s);
 int launch_frac_sum(const void *img, void *out, int64_t n, int64_t npix, int f64, hipStream_t s);
 int launch_adaptive_mask(const void *img, double *mask, int64_t n_img, int64_t H, int64_t W, int fill_hole,
                          int dilation, int erosion, int f64, int *status, void *thresholds, hipStream_t s);
@@ -602,6 +607,26 @@ int fl_frac_final(const double *tmp, const double *mask, const double *ratio, co
     if (n == 0 || npix == 0) return FL_OK;
     FL_REQUIRE(tmp && lohi && frac, "null pointer argument");
     return launch_frac_final(tmp, mask, ratio, lohi, frac, n, npix, single != 0, (hipStream_t)stream);
+}
+
+size_t fl_cone_sweep_workspace_bytes(int64_t K, int64_t Z, int64_t X, int64_t Y, const int32_t *pz,
+                                     const int32_t *py, const int32_t *q, int64_t ndir)
+{
+    if (K < 1 || Z < 1 || X < 1 || Y < 1 || ndir < 1 || !pz || !py || !q) return 0;
+    return cone_sweep_workspace_bytes(K, Z, X, Y, pz, py, q, ndir);
+}
+
+int fl_cone_sweep(const float *mu_rot, float *T, int64_t K, int64_t Z, int64_t X, int64_t Y, const int32_t *pz,
+                  const int32_t *py, const int32_t *q, const float *weight, int64_t ndir, int adjoint,
+                  void *workspace, size_t workspace_bytes, void *stream)
+{
+    g_last_error.clear();
+    FL_REQUIRE(K >= 1 && Z >= 1 && X >= 1 && Y >= 1 && ndir >= 1, "shape: bad size");
+    FL_REQUIRE(Z < (1 << 20) && X < (1 << 20) && Y < (1 << 20) && K < 2048, "shape: too large");
+    FL_REQUIRE(mu_rot && T && pz && py && q && weight, "null pointer argument");
+    FL_REQUIRE(mu_rot != T, "T must not alias mu");
+    return launch_cone_sweep(mu_rot, T, K, Z, X, Y, pz, py, q, weight, ndir, adjoint != 0, workspace,
+                             workspace_bytes, (hipStream_t)stream);
 }
 
 }  // extern "C"

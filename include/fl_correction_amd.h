@@ -301,6 +301,23 @@ FL_API int fl_cone_atten_adjoint(const float *g_rot, float *dmu_rot, int64_t K, 
                                  const int32_t *q, const float *weight, int64_t ndir,
                                  void *stream);
 
+/* fl_cone_atten (adjoint = 0) or fl_cone_atten_adjoint (adjoint = 1) as one sweep
+ * over blocks of Q planes for all directions at once: T is written once instead of
+ * read and rewritten per direction.  Every direction's denominator q[d] must divide
+ * one Q in {4, 8, 16} (Q = max q[d]); at most 64 directions; the results equal the
+ * per-direction form to rounding (the ray sums are split every Q steps instead of
+ * every reduced q[d]).  Returns FL_ERR_UNSUPPORTED for other direction sets.  The
+ * workspace (fl_cone_sweep_workspace_bytes, 0 when unsupported) holds each
+ * direction's running ray sums of one block, twice.  Direction arrays are HOST
+ * memory.  No reference counterpart (the reference has only the dense mask form). */
+FL_API size_t fl_cone_sweep_workspace_bytes(int64_t K, int64_t Z, int64_t X, int64_t Y,
+                                            const int32_t *pz, const int32_t *py, const int32_t *q,
+                                            int64_t ndir);
+FL_API int fl_cone_sweep(const float *mu_rot, float *T, int64_t K, int64_t Z, int64_t X, int64_t Y,
+                         const int32_t *pz, const int32_t *py, const int32_t *q, const float *weight,
+                         int64_t ndir, int adjoint, void *workspace, size_t workspace_bytes,
+                         void *stream);
+
 /* ---------------------------------------------------------------------------
  * Adjoint of the attenuated emission model.  The reference differentiates the
  * projector with respect to C only, holding the attenuation fixed
