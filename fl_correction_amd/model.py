@@ -73,6 +73,7 @@ class XRFModel:
         self.store = store
         self.memory_fraction = memory_fraction
         self._T_store = None
+        self._sweep_ws = {}
 
     # ------------------------------------------------------------------
     def _st(self):
@@ -81,11 +82,35 @@ class XRFModel:
     def _dir_ptrs(self):
         return [d.ctypes.data_as(_lib.ctypes.c_void_p) for d in self._dirs]
 
+    def _sweep_workspace(self, shape):
+        """Workspace of the all-directions cone sweep for the current stream (one per
+        stream: concurrent angles must not share its ring), or None when the sweep
+        does not apply to this direction set."""
+        K, B, H, W = shape
+        st = self._st()
+        key = (int(getattr(st, "value", st) or 0), K, B, H, W)
+        ws = self._sweep_ws.get(key)
+        if ws is None:
+            nb = int(_lib.load().fl_cone_sweep_workspace_bytes(K, B, W, H, *self._dir_ptrs()[:3],
+                                                               self.detector.ndir))
+            ws = False if nb == 0 else torch.empty((nb + 3) // 4, dtype=torch.float32, device=self.device)
+            self._sweep_ws[key] = ws
+        return None if ws is False else ws
+
+    def _cone(self, x, out, shape, adjoint):
+        K, B, H, W = shape
+        ws = self._sweep_workspace(shape)
+        if ws is not None:
+            _lib.call("fl_cone_sweep", _lib.ptr(x), _lib.ptr(out), K, B, W, H, *self._dir_ptrs(), self.detector.ndir,
+                      int(adjoint), _lib.ptr(ws), ws.numel() * 4, self._st())
+        else:
+            fn = "fl_cone_atten_adjoint" if adjoint else "fl_cone_atten"
+            _lib.call(fn, _lib.ptr(x), _lib.ptr(out), K, B, W, H, *self._dir_ptrs(), self.detector.ndir, self._st())
+
     def _exponent(self, mu_rot, T_rot, shape):
         K, B, H, W = shape
         if isinstance(self.detector, Cone):
-            _lib.call("fl_cone_atten", _lib.ptr(mu_rot), _lib.ptr(T_rot), K, B, W, H, *self._dir_ptrs(),
-                      self.detector.ndir, self._st())
+            self._cone(mu_rot, T_rot, shape, adjoint=False)
         else:
             Mz, Mx, My = self.mask.shape
             _lib.call("fl_cal_atten", _lib.ptr(mu_rot), _lib.ptr(self.mask), _lib.ptr(T_rot), K, B, W, H, Mz, Mx, My,
@@ -94,8 +119,7 @@ class XRFModel:
     def _exponent_adjoint(self, g_rot, out, shape):
         K, B, H, W = shape
         if isinstance(self.detector, Cone):
-            _lib.call("fl_cone_atten_adjoint", _lib.ptr(g_rot), _lib.ptr(out), K, B, W, H, *self._dir_ptrs(),
-                      self.detector.ndir, self._st())
+            self._cone(g_rot, out, shape, adjoint=True)
         else:
             Mz, Mx, My = self.mask.shape
             _lib.call("fl_cal_atten_adjoint", _lib.ptr(g_rot), _lib.ptr(self.mask), _lib.ptr(out), K, B, W, H, Mz, Mx,

@@ -1,7 +1,7 @@
 """Drop-in for the attenuation and projection helpers of FL/torch_functions.py.
 
 torch_xray_atten_3D  -> fl_incident_atten (HIP)
-torch_xrf_atten_3D   -> fl_mu_xrf_rot + (fl_cal_atten | fl_cone_atten) + fl_xrf_combine (HIP)
+torch_xrf_atten_3D   -> fl_mu_xrf_rot + (fl_cal_atten | fl_cone_sweep | fl_cone_atten) + fl_xrf_combine (HIP)
 torch_3D_forward_projection -> the projector op set
 
 ``detector_mask`` may be a dense mask array (the reference's form: correlated
@@ -91,11 +91,16 @@ def xrf_exponent_rot(frac, coef, detector_mask, side=0, work=None):
         _lib.call("fl_mu_xrf_rot", _lib.ptr(frac), _lib.ptr(coef_d), _lib.ptr(mu_rot), Kc, K, B, H, W, side, st)
         if isinstance(detector_mask, Cone):
             cone = detector_mask
-            pz, py, qd, wt = (np.ascontiguousarray(a) for a in (cone.pz, cone.py, cone.qd, cone.weight))
-            _lib.call("fl_cone_atten", _lib.ptr(mu_rot), _lib.ptr(T), K, B, W, H,
-                      pz.ctypes.data_as(_lib.ctypes.c_void_p), py.ctypes.data_as(_lib.ctypes.c_void_p),
-                      qd.ctypes.data_as(_lib.ctypes.c_void_p), wt.ctypes.data_as(_lib.ctypes.c_void_p),
-                      cone.ndir, st)
+            arrs = [np.ascontiguousarray(a) for a in (cone.pz, cone.py, cone.qd, cone.weight)]
+            ptrs = [a.ctypes.data_as(_lib.ctypes.c_void_p) for a in arrs]
+            # all directions in one sweep where it applies, else direction by direction
+            nb = int(_lib.load().fl_cone_sweep_workspace_bytes(K, B, W, H, *ptrs[:3], cone.ndir))
+            if nb:
+                ws = torch.empty((nb + 3) // 4, dtype=torch.float32, device=dev)
+                _lib.call("fl_cone_sweep", _lib.ptr(mu_rot), _lib.ptr(T), K, B, W, H, *ptrs, cone.ndir, 0,
+                          _lib.ptr(ws), ws.numel() * 4, st)
+            else:
+                _lib.call("fl_cone_atten", _lib.ptr(mu_rot), _lib.ptr(T), K, B, W, H, *ptrs, cone.ndir, st)
         else:
             mask = _as_device_f32(detector_mask, dev)
             if mask.dim() != 3:
