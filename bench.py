@@ -21,8 +21,8 @@ Besides the timed steps, the JSON line carries
   * roofline: the dominant kernel's algorithmic HBM bytes over its average
     duration, timed with HIP events on the stream it is launched on;
   * model: attenuation, forward and gradient (vjp) of the differentiable model
-    with a 5x5 detector cone over 15 deg, timed on a subset of the angles and
-    extrapolated per angle to all 360;
+    with a 5x5 detector cone over 15 deg, timed over all 360 angles, and the
+    cone march's roofline (flop count and its own HBM traffic);
   * cpu_baseline: the oracle's CPU restatement of the same iteration, timed on
     one line on the host cores and scaled to all lines.
 
@@ -40,6 +40,7 @@ import numpy as np
 
 ROOT = os.path.dirname(os.path.abspath(__file__))
 HBM_PEAK_GBS = 8000.0
+FP32_PEAK_TFLOPS = 157.3
 
 
 def parse():
@@ -52,8 +53,9 @@ def parse():
     ap.add_argument("--elements", type=int, default=4)
     ap.add_argument("--cpu-angles", type=int, default=240,
                     help="angles of one line timed on the CPU for the baseline (scaled to the full step)")
-    ap.add_argument("--model-angles", type=int, default=8,
-                    help="angles of the differentiable model timed (0: skip), extrapolated per angle")
+    ap.add_argument("--model-angles", type=int, default=-1,
+                    help="angles of the differentiable model timed (-1: all of --angles, measured; "
+                         "fewer: extrapolated per angle; 0: skip)")
     ap.add_argument("--batch-samples", type=int, default=48,
                     help="samples of the batched uneven-instance workload (0: skip)")
     ap.add_argument("--batch-iters", type=int, default=20)
@@ -277,7 +279,7 @@ def main():
     del solvers, C, Cn, I, atten
     torch.cuda.empty_cache()
 
-    if args.model_angles > 0:
+    if args.model_angles != 0:
         result["model"] = model_timings(args, C_true, pix)
 
     if args.batch_samples > 0:
@@ -290,14 +292,16 @@ def main():
             # the iteration that recomputes the attenuation for every angle first
             m = result["model"]
             cpu_att = cpu_attenuation_sample(args)
-            gpu_ms = ms_per_step + A * m["attenuation_ms_per_angle"]
+            gpu_ms = ms_per_step + (m["attenuation_ms_all_angles"] if m.get("attenuation_ms_all_angles")
+                                    else A * m["attenuation_ms_per_angle"])
             cpu_s = 1.0 / result["cpu_baseline"]["value"] + A * cpu_att["s_per_angle"]
             result["iteration_with_attenuation"] = {
                 "gpu_ms": gpu_ms, "cpu_s": cpu_s, "speedup": cpu_s * 1e3 / gpu_ms,
                 "cpu_attenuation": cpu_att,
                 "note": "MLEM step plus attenuation maps of all lines at every angle (5x5 cone over 15 deg); "
-                        "GPU attenuation per angle from the model timings, CPU from the oracle's direct FP64 march "
-                        "on a slab, both scaled per angle"}
+                        "GPU attenuation timed over all angles in the model timings (per angle scaled when "
+                        "--model-angles is smaller), CPU from the oracle's direct FP64 march on a slab, scaled "
+                        "per angle"}
 
     print(json.dumps(result), flush=True)
 
@@ -319,10 +323,11 @@ def measured_traffic(kernel, N, A):
 
 
 def model_timings(args, C_true, pix):
-    """Attenuation, forward and vjp of the differentiable model (XRFModel) on a
-    subset of the angles, per angle; the iteration and gradient figures are the
-    per-angle times times the full angle count."""
+    """Attenuation, forward and vjp of the differentiable model (XRFModel) over all
+    the angles (or, with --model-angles N, over N of them, the per-angle times then
+    scaled to all angles and marked extrapolated), and the cone march's roofline."""
     import torch
+    from fl_correction_amd import _lib
     from fl_correction_amd.atten_util import make_param
     from fl_correction_amd.detector_mask import Cone
     from fl_correction_amd.model import XRFModel
@@ -332,14 +337,15 @@ def model_timings(args, C_true, pix):
     cs_x = rng.uniform(20, 400, K)
     cs_line = rng.uniform(20, 400, (K, K))
     param = make_param([f"E{i}" for i in range(K)], [f"C{i}" for i in range(K)], cs_x, cs_line, rho=1.0, pix=pix)
-    na = min(args.model_angles, A)
+    na = A if args.model_angles < 0 else min(args.model_angles, A)
+    measured = na == A
     angles = np.linspace(0.0, 180.0, na, endpoint=False)
     cone = Cone(n=5, half_angle_deg=15.0, q=8)
-    model = XRFModel(param, angles, cone, store=False)
     Cm = (C_true * 0.5).contiguous()
 
-    def timed(fn):
-        fn()
+    def timed(fn, warm=None):
+        # warm-up on a few angles' model (kernels, workspaces), then the timed call
+        (warm or fn)()
         torch.cuda.synchronize()
         e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
         e0.record()
@@ -348,45 +354,107 @@ def model_timings(args, C_true, pix):
         torch.cuda.synchronize()
         return e0.elapsed_time(e1), out
 
-    t_att, att = timed(lambda: model.attenuation(Cm))
+    wmodel = XRFModel(param, angles[:4], cone, store=False)
+    Gw = torch.randn_like(wmodel.forward(Cm))
+    model = XRFModel(param, angles, cone, store=False)
+    t_att, att = timed(lambda: model.attenuation(Cm), lambda: wmodel.attenuation(Cm))
     del att
-    t_fwd, Pf = timed(lambda: model.forward(Cm))
+    t_fwd, Pf = timed(lambda: model.forward(Cm), lambda: wmodel.forward(Cm))
     G = torch.randn_like(Pf)
-    t_vjp, _ = timed(lambda: model.vjp(Cm, G))
+    t_vjp, _ = timed(lambda: model.vjp(Cm, G), lambda: wmodel.vjp(Cm, Gw))
     model.release()
     del model
     # The same with the fluorescence exponents kept from forward to vjp.
     smodel = XRFModel(param, angles, cone, store=True)
-    t_sfwd, _ = timed(lambda: smodel.forward(Cm))
-    t_svjp, _ = timed(lambda: smodel.vjp(Cm, G))
+    t_sfwd, _ = timed(lambda: smodel.forward(Cm), lambda: wmodel.forward(Cm))
+    t_svjp, _ = timed(lambda: smodel.vjp(Cm, G), lambda: wmodel.vjp(Cm, Gw))
     smodel.release()
-    del smodel
+    del smodel, wmodel
+    torch.cuda.empty_cache()
     per = lambda t: t / na
     log(f"model: {na} angles: attenuation {t_att:.1f} ms, forward {t_fwd:.1f} ms, vjp {t_vjp:.1f} ms; "
         f"exponents kept: forward {t_sfwd:.1f} ms, vjp {t_svjp:.1f} ms")
     B, H, W = Cm.shape[1:]
     store_bytes = 4.0 * A * K * B * H * W  # all angles' exponents
     total = torch.cuda.get_device_properties(0).total_memory
+    grad = {"grad_eval_ms": t_fwd + t_vjp, "grad_evals_per_s": 1000.0 / (t_fwd + t_vjp)} if measured else \
+        {"grad_eval_ms_extrapolated": per(t_fwd + t_vjp) * A,
+         "grad_evals_per_s_extrapolated": 1000.0 / (per(t_fwd + t_vjp) * A)}
+    kept = {"grad_eval_ms": t_sfwd + t_svjp, "grad_evals_per_s": 1000.0 / (t_sfwd + t_svjp)} if measured else \
+        {"grad_evals_per_s_extrapolated": 1000.0 / (per(t_sfwd + t_svjp) * A)}
     return {
-        "angles_timed": na, "cone": "5x5 directions, 15 deg half-angle, slopes in 1/8",
+        "angles_timed": na, "measured_over_all_angles": measured,
+        "cone": "5x5 directions, 15 deg half-angle, slopes in 1/8",
         "attenuation_ms_per_angle": per(t_att),
+        "attenuation_ms_all_angles": t_att if measured else None,
         "forward_ms_per_angle": per(t_fwd),
         "vjp_ms_per_angle": per(t_vjp),
         "adjoint_over_forward": t_vjp / t_fwd,
-        "grad_eval_ms_extrapolated": per(t_fwd + t_vjp) * A,
-        "grad_evals_per_s_extrapolated": 1000.0 / (per(t_fwd + t_vjp) * A),
+        **grad,
         "note": "forward recomputes the attenuation for every angle (nothing stored); vjp recomputes it too "
                 "and differentiates through it",
         "exponents_kept": {
             "forward_ms_per_angle": per(t_sfwd),
             "vjp_ms_per_angle": per(t_svjp),
             "adjoint_over_forward": t_svjp / t_sfwd,
-            "grad_evals_per_s_extrapolated": 1000.0 / (per(t_sfwd + t_svjp) * A),
+            **kept,
             "stored_gb_at_all_angles": store_bytes / 1e9,
             "fits_default_budget": bool(store_bytes <= 0.4 * total),
             "note": "forward keeps every angle's fluorescence exponents and vjp reuses them (the model's "
                     "store mode; its default 'auto' keeps them when they fit 40% of free HBM)",
         },
+        "cone_roofline": cone_roofline(K, B, H, W, cone),
+    }
+
+
+def cone_roofline(K, B, H, W, cone, reps=5):
+    """The cone march of one angle for all lines (fl_cone_sweep on the mask frame
+    (K, B, W, H)), timed with HIP events on its stream, against both roofs: the
+    ray-form flop count 2*K*N^3*sum_d taps_d (taps_d = the bilinear taps of
+    direction d's Q-step stencil), and the sweep's own HBM traffic (mu read once,
+    T written once, each direction's running ray sum written and read once)."""
+    import torch
+    from fl_correction_amd import _lib
+    dev = torch.device("cuda:0")
+    arrs = [np.ascontiguousarray(a) for a in (cone.pz, cone.py, cone.qd, cone.weight)]
+    ptrs = [a.ctypes.data_as(_lib.ctypes.c_void_p) for a in arrs]
+    Z, X, Y = B, W, H
+    nb = int(_lib.load().fl_cone_sweep_workspace_bytes(K, Z, X, Y, *ptrs[:3], cone.ndir))
+    mu = torch.rand((K, Z, X, Y), device=dev) * 0.01
+    T = torch.empty_like(mu)
+    ws = torch.empty((nb + 3) // 4, device=dev)
+    st = _lib.stream_of(dev)
+    run = lambda: _lib.call("fl_cone_sweep", _lib.ptr(mu), _lib.ptr(T), K, Z, X, Y, *ptrs, cone.ndir, 0,
+                            _lib.ptr(ws), ws.numel() * 4, st)
+    run()
+    torch.cuda.synchronize()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for _ in range(reps):
+        run()
+    e1.record()
+    torch.cuda.synchronize()
+    ms = e0.elapsed_time(e1) / reps
+    Q = int(max(cone.qd))
+    taps = 0
+    for pz, py, q in zip(cone.pz, cone.py, cone.qd):
+        m = Q // int(q)
+        for t in range(1, Q + 1):
+            fz = (t * int(pz) * m) % Q != 0
+            fy = (t * int(py) * m) % Q != 0
+            taps += (2 if fz else 1) * (2 if fy else 1)
+    vox = K * Z * X * Y
+    flops = 2.0 * vox * taps
+    traffic = 4.0 * vox * (2 + 2 * cone.ndir)
+    del mu, T, ws
+    return {
+        "kernel": "cone_sweep_kernel<8,8> (X/Q launches: one angle, all lines)", "ms_per_angle": ms,
+        "taps_per_voxel": taps,
+        "flops": flops, "achieved_tflops": flops / ms / 1e9, "fp32_peak_tflops": FP32_PEAK_TFLOPS,
+        "flop_frac": flops / ms / 1e9 / FP32_PEAK_TFLOPS,
+        "traffic_bytes": traffic, "achieved_gbs": traffic / ms / 1e6, "hbm_peak_gbs": HBM_PEAK_GBS,
+        "hbm_frac": traffic / ms / 1e6 / HBM_PEAK_GBS,
+        "bound": "hbm (the running ray sums: 8 bytes per voxel and direction)",
     }
 
 
