@@ -362,8 +362,12 @@ def model_timings(args, C_true, pix):
     t_fwd, Pf = timed(lambda: model.forward(Cm), lambda: wmodel.forward(Cm))
     G = torch.randn_like(Pf)
     t_vjp, _ = timed(lambda: model.vjp(Cm, G), lambda: wmodel.vjp(Cm, Gw))
+    # Poisson loss, gradient and projections in one pass (what joint reconstruction runs)
+    Iw = torch.rand_like(Gw) + 0.5
+    Im = Pf * (torch.rand_like(Pf) * 0.4 + 0.8)
+    t_lg, _ = timed(lambda: model.loss_and_grad(Cm, Im), lambda: wmodel.loss_and_grad(Cm, Iw))
     model.release()
-    del model
+    del model, Im
     # The same with the fluorescence exponents kept from forward to vjp.
     smodel = XRFModel(param, angles, cone, store=True)
     t_sfwd, _ = timed(lambda: smodel.forward(Cm), lambda: wmodel.forward(Cm))
@@ -372,14 +376,17 @@ def model_timings(args, C_true, pix):
     del smodel, wmodel
     torch.cuda.empty_cache()
     per = lambda t: t / na
-    log(f"model: {na} angles: attenuation {t_att:.1f} ms, forward {t_fwd:.1f} ms, vjp {t_vjp:.1f} ms; "
-        f"exponents kept: forward {t_sfwd:.1f} ms, vjp {t_svjp:.1f} ms")
+    log(f"model: {na} angles: attenuation {t_att:.1f} ms, forward {t_fwd:.1f} ms, vjp {t_vjp:.1f} ms, "
+        f"loss and gradient in one pass {t_lg:.1f} ms; exponents kept: forward {t_sfwd:.1f} ms, "
+        f"vjp {t_svjp:.1f} ms")
     B, H, W = Cm.shape[1:]
     store_bytes = 4.0 * A * K * B * H * W  # all angles' exponents
     total = torch.cuda.get_device_properties(0).total_memory
     grad = {"grad_eval_ms": t_fwd + t_vjp, "grad_evals_per_s": 1000.0 / (t_fwd + t_vjp)} if measured else \
         {"grad_eval_ms_extrapolated": per(t_fwd + t_vjp) * A,
          "grad_evals_per_s_extrapolated": 1000.0 / (per(t_fwd + t_vjp) * A)}
+    fused = {"grad_eval_ms": t_lg, "grad_evals_per_s": 1000.0 / t_lg} if measured else \
+        {"grad_eval_ms_extrapolated": per(t_lg) * A, "grad_evals_per_s_extrapolated": 1000.0 / (per(t_lg) * A)}
     kept = {"grad_eval_ms": t_sfwd + t_svjp, "grad_evals_per_s": 1000.0 / (t_sfwd + t_svjp)} if measured else \
         {"grad_evals_per_s_extrapolated": 1000.0 / (per(t_sfwd + t_svjp) * A)}
     return {
@@ -393,6 +400,14 @@ def model_timings(args, C_true, pix):
         **grad,
         "note": "forward recomputes the attenuation for every angle (nothing stored); vjp recomputes it too "
                 "and differentiates through it",
+        "loss_and_grad": {
+            "ms_per_angle": per(t_lg),
+            **fused,
+            "note": "Poisson loss, its gradient and the projections in one pass over the angles "
+                    "(XRFModel.loss_and_grad, what joint reconstruction runs): each chunk of 16 angles is "
+                    "projected, its cotangent formed and its adjoint run on the chunk's exponents, so the "
+                    "cone runs twice per angle",
+        },
         "exponents_kept": {
             "forward_ms_per_angle": per(t_sfwd),
             "vjp_ms_per_angle": per(t_svjp),

@@ -7,7 +7,8 @@ runs forward and vjp on `--timed-angles` of the `--angles` angles (evenly
 spread over 0-360 degrees, or 0-180 with --half-turn), with the fluorescence
 exponents recomputed per angle (store=False) unless --store is given, and
 prints per-angle times, the ratio vjp / forward, the time of one full gradient
-evaluation (per-angle time x all angles: angles are independent) and the peak
+evaluation (per-angle time x all angles: angles are independent), the same for
+the one-pass Poisson loss and gradient (XRFModel.loss_and_grad) and the peak
 device memory.  Inputs: a seeded ellipsoid phantom and seeded cross-sections.
 """
 from __future__ import annotations
@@ -64,7 +65,10 @@ def main(argv=None):
     t_f, Pf = timed(lambda: model.forward(C))
     G = torch.randn_like(Pf)
     t_b, g = timed(lambda: model.vjp(C, G))
-    ok = bool(torch.isfinite(Pf).all()) and bool(torch.isfinite(g).all())
+    I = Pf * (torch.rand_like(Pf) * 0.4 + 0.8)
+    model.loss_and_grad(C, I)  # warm-up
+    t_lg, (loss, g2, _) = timed(lambda: model.loss_and_grad(C, I))
+    ok = all(bool(torch.isfinite(x).all()) for x in (Pf, g, g2, loss))
     model.release()
     exponent_gb = 4.0 * A * K * B * N * N / 1e9
     print(json.dumps({
@@ -73,6 +77,8 @@ def main(argv=None):
         "exponents": "kept" if args.store else "recomputed per angle",
         "forward_ms_per_angle": t_f / na, "vjp_ms_per_angle": t_b / na, "vjp_over_forward": t_b / t_f,
         "grad_eval_s": (t_f + t_b) / na * A / 1e3,
+        "loss_and_grad_ms_per_angle": t_lg / na,
+        "loss_and_grad_eval_s": t_lg / na * A / 1e3,
         "exponents_gb_all_angles": exponent_gb,
         "peak_memory_gb": torch.cuda.max_memory_allocated() / 1e9,
         "finite": ok,

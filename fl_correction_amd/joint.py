@@ -18,7 +18,8 @@ Optimiser: Adam with a projection onto C >= 0 after every step.  The start is
 an MLEM estimate (``mlem_start``): per emission line, ``MLEMSolver`` on the
 attenuation of the initial estimate.
 
-Every model evaluation runs the library's kernels (forward, vjp); the loss and
+Every model evaluation runs the library's kernels (``XRFModel.loss_and_grad``:
+forward, the loss's cotangent and the adjoint per chunk of angles); the loss and
 the Adam update are elementwise torch operations on the device.
 """
 from __future__ import annotations
@@ -125,13 +126,13 @@ def joint_reconstruct(model: XRFModel, I, C0, n_steps=100, lr=None, eps=1e-6, ca
     opt = torch.optim.Adam([C], lr=lr)
     history = []
     for step in range(int(n_steps) + 1):
-        opt.zero_grad(set_to_none=True)
-        Pf = XRFModelFunction.apply(C, model, streams)
-        loss = poisson_loss(Pf, I, eps)
-        history.append((step, float(loss.detach()), data_residual(Pf.detach(), I)))
+        # loss, gradient and projections in one pass over the angles (the cone
+        # twice per angle instead of three times for forward then vjp)
+        loss, grad, Pf = model.loss_and_grad(C, I, eps, streams=streams)
+        history.append((step, float(loss), data_residual(Pf, I)))
         if step == n_steps:
             break
-        loss.backward()
+        C.grad = grad
         opt.step()
         with torch.no_grad():
             C.clamp_(min=0.0)

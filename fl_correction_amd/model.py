@@ -296,7 +296,7 @@ class XRFModel:
                 for sx in sts[1:]:
                     sx.wait_event(ready)
                 try:
-                    self._vjp_angles(C, G, a0, a1, n_st, sts, bufs, acc, Atot, stored)
+                    self._vjp_angles(C, G, a0, a1, n_st, sts, bufs, acc, Atot, self._T_store if stored else None, 0)
                 finally:
                     self._join(cur, sts[1:])
                 for e in range(K):
@@ -307,16 +307,17 @@ class XRFModel:
                 dC.add_(extra)  # on the caller's stream, after it has waited for the side streams
         return dC
 
-    def _vjp_angles(self, C, G, a0, a1, n_st, sts, bufs, acc, Atot, stored):
-        """The per-angle adjoints of angles a0..a1-1, dealt round-robin to the streams."""
+    def _vjp_angles(self, C, G, a0, a1, n_st, sts, bufs, acc, Atot, T_keep, t0):
+        """The per-angle adjoints of angles a0..a1-1, dealt round-robin to the streams.
+        Angle a's exponent is T_keep[a - t0] when T_keep is given, else recomputed."""
         K, B, H, W = C.shape
         for a in range(a0, a1):
             i = (a - a0) % n_st
             with torch.cuda.stream(sts[i]):
                 sa = self._st()
                 F, Ax, gA, gx, dmux, dF, mu_rot, T_tmp, dT_rot, dmu_rot = bufs[i]
-                T_rot = self._T_store[a] if stored else T_tmp
-                self._angle_maps(C, a, F, Ax, mu_rot, T_rot, recompute_T=not stored)
+                T_rot = T_tmp if T_keep is None else T_keep[a - t0]
+                self._angle_maps(C, a, F, Ax, mu_rot, T_rot, recompute_T=T_keep is None)
                 for e in range(K):
                     _lib.call("fl_sample_grad", _lib.ptr(C[e]), _lib.ptr(self.em_T[e, a:]),
                               _lib.ptr(self.theta[a:]), _lib.ptr(G[e, a]), _lib.ptr(gA[e]), 1, B, 1, H, W, 0, sa)
@@ -329,6 +330,87 @@ class XRFModel:
                           _lib.ptr(self.coef_x[a]), _lib.ptr(dF), K, K, B, H, W, self.side, sa)
                 c, s = self.trig[a]
                 _lib.call("fl_rotate_adjoint", _lib.ptr(dF), _lib.ptr(acc[i]), K * B, H, W, c, s, 1, sa)
+
+    def loss_and_grad(self, C, I, eps=1e-6, angle_chunk=16, streams=2):
+        """Poisson loss mean(P - I log P), P = max(Pf, eps) (FL/ml_lib/ml_loss.py:6-8),
+        its gradient dL/dC and the projections Pf, in one pass over the angles.
+
+        The loss's cotangent dL/dPf at angle a depends on angle a's projections
+        only, so each chunk of angles is projected, its cotangent formed and its
+        adjoint run while the chunk's fluorescence exponents are still resident:
+        the cone runs twice per angle (exponent, then its adjoint), where forward
+        followed by vjp runs it three times unless every angle's exponent is kept.
+        Memory: one chunk of exponents and attenuation maps, not one per angle.
+        Returns (loss as a 0-d tensor, dL/dC (K, B, H, W), Pf (K, A, B, W)); they
+        equal poisson_loss(forward(C), I) and vjp's result to rounding."""
+        C = C.detach().to(self.device, torch.float32).contiguous()
+        I = I.to(self.device, torch.float32).contiguous()
+        K, B, H, W = C.shape
+        if K != self.K:
+            raise RuntimeError(f"C has {K} element maps, the model {self.K}")
+        if tuple(I.shape) != (K, self.A, B, W):
+            raise RuntimeError(f"I must be {(K, self.A, B, W)}")
+        nA = min(angle_chunk, self.A)
+        n_st = max(1, min(int(streams), nA))
+        Pf = torch.empty((K, self.A, B, W), dtype=torch.float32, device=self.device)
+        G = torch.empty_like(Pf)
+        Atot = torch.empty((K, nA, B, H, W), dtype=torch.float32, device=self.device)
+        T_chunk = torch.empty((nA, K, B, W, H), dtype=torch.float32, device=self.device)
+        acc = [torch.zeros_like(C) for _ in range(n_st)]  # acc[0] is the result
+        bufs = []
+        for _ in range(n_st):
+            F, Ax, gA, gx, dmux, dF = (torch.empty_like(C) for _ in range(6))
+            mu_rot = torch.empty((K, B, W, H), dtype=torch.float32, device=self.device)
+            bufs.append((F, Ax, gA, gx, dmux, dF, mu_rot, torch.empty_like(mu_rot), torch.empty_like(mu_rot),
+                         torch.empty_like(mu_rot)))
+        inv_n = 1.0 / Pf.numel()
+        st = self._st()
+        dC = acc[0]
+        with torch.cuda.device(self.device):
+            cur = torch.cuda.current_stream(self.device)
+            sts = [cur] + self._side_streams(n_st - 1)
+
+            def fork():
+                ready = torch.cuda.Event()
+                ready.record(cur)
+                for sx in sts[1:]:
+                    sx.wait_event(ready)
+
+            for a0 in range(0, self.A, nA):
+                a1 = min(a0 + nA, self.A)
+                fork()  # the previous chunk's emission adjoint (reads Atot) is done
+                try:
+                    for a in range(a0, a1):
+                        i = (a - a0) % n_st
+                        with torch.cuda.stream(sts[i]):
+                            F, Ax, mu_rot = bufs[i][0], bufs[i][1], bufs[i][6]
+                            T_rot = T_chunk[a - a0]
+                            self._angle_maps(C, a, F, Ax, mu_rot, T_rot)
+                            _lib.call("fl_xrf_combine", _lib.ptr(T_rot), _lib.ptr(Ax), _lib.ptr(Atot[0, a - a0]), K,
+                                      B, H, W, B * H * W, Atot.stride(0), self.side, self._st())
+                finally:
+                    self._join(cur, sts[1:])
+                for e in range(K):
+                    _lib.call("fl_forward_emission_batch", _lib.ptr(Atot[e]), _lib.ptr(C[e]), _lib.ptr(self.em_T[e, a0:]),
+                              _lib.ptr(self.theta[a0:]), _lib.ptr(Pf[e, a0]), a1 - a0, B, 1, H, W, st)
+                # dL/dPf of the chunk, as autograd forms it through clamp_min, log and mean
+                Pc, Ic = Pf[:, a0:a1], I[:, a0:a1]
+                P = Pc.clamp_min(eps)
+                G[:, a0:a1] = torch.where(Pc >= eps, inv_n - (Ic * inv_n) / P, torch.zeros((), device=self.device))
+                fork()
+                try:
+                    self._vjp_angles(C, G, a0, a1, n_st, sts, bufs, acc, Atot, T_chunk, a0)
+                finally:
+                    self._join(cur, sts[1:])
+                for e in range(K):
+                    _lib.call("fl_backward_emission_accumulate", _lib.ptr(Atot[e]), _lib.ptr(G[e, a0]),
+                              _lib.ptr(self.em_T[e, a0:]), _lib.ptr(self.theta[a0:]), _lib.ptr(dC[e]), a1 - a0, B, 1,
+                              H, W, st)
+            for extra in acc[1:]:
+                dC.add_(extra)
+            P = Pf.clamp_min(eps)
+            loss = (P - I * torch.log(P)).mean()
+        return loss, dC, Pf
 
     def release(self):
         self._T_store = None

@@ -182,3 +182,47 @@ def test_stored_exponents_over_streams():
     assert torch.equal(Pf.detach(), P1)
     (Pf * G).sum().backward()
     assert torch.equal(Cg.grad, one.vjp(Cd, G, streams=1))
+
+
+@pytest.mark.parametrize("kind", ["cone", "mask"])
+@pytest.mark.parametrize("streams", [1, 2])
+def test_loss_and_grad_matches_forward_then_vjp(kind, streams):
+    """The one-pass loss, gradient and projections (chunks of 2 of 5 angles, the
+    last one partial) against the Poisson loss of the forward and the gradient
+    through the autograd function (forward, then vjp with the exponents
+    recomputed), and the loss and gradient against torch autograd through the
+    FP64 torch model.  Some projections lie below eps, so the clamp's zero
+    derivative is exercised."""
+    from fl_correction_amd.joint import poisson_loss
+    from fl_correction_amd.model import XRFModel, xrf_forward_autograd
+    C, param, angles, det, mask, em, coef_x, coef_xrf = _model_case(kind, seed=31)
+    model = XRFModel(param, angles, det, em=em, store=False)
+    Cd = _dev(C)
+    Pf0 = model.forward(Cd)
+    # 10% of the projections clamped, eps half-way between two projections so that
+    # the FP64 model clamps the same ones
+    srt = Pf0.flatten().sort().values
+    k = srt.numel() // 10
+    eps = 0.5 * float(srt[k - 1] + srt[k])
+    I = Pf0 * torch.from_numpy(np.random.default_rng(32).uniform(0.7, 1.3, Pf0.shape)).float().cuda()
+    loss, g, Pf = model.loss_and_grad(Cd, I, eps=eps, angle_chunk=2, streams=streams)
+    assert torch.equal(Pf, Pf0)
+    Cg = Cd.clone().requires_grad_(True)
+    ref_loss = poisson_loss(xrf_forward_autograd(Cg, model, streams=streams), I, eps)
+    ref_loss.backward()
+    e_l = abs(float(loss) - float(ref_loss)) / abs(float(ref_loss))
+    e_g = rel_l2(g.cpu().numpy(), Cg.grad.cpu().numpy())
+    # FP64 torch model and autograd
+    Ct = torch.tensor(C, dtype=torch.float64, requires_grad=True)
+    P64 = tm.forward(Ct, angles, coef_x, coef_xrf, em, mask).clamp_min(eps)
+    I64 = I.double().cpu()
+    L64 = (P64 - I64 * torch.log(P64)).mean()
+    L64.backward()
+    e_l64 = abs(float(loss) - float(L64)) / abs(float(L64))
+    e_g64 = rel_l2(g.cpu().numpy(), Ct.grad.numpy())
+    print(f"loss_and_grad {kind} streams={streams}: vs forward+vjp loss {e_l:.1e} grad {e_g:.1e}; "
+          f"vs torch f64 loss {e_l64:.1e} grad {e_g64:.1e}")
+    assert e_l <= 1e-6
+    assert e_g <= 1e-5
+    assert e_l64 <= 1e-5
+    assert e_g64 <= 1e-4
