@@ -63,6 +63,7 @@ SIGNATURES = {
     "fl_mu_adjoint": (_INT, [_P, _P, _P, _P, _P, _I64, _I64, _I64, _I64, _I64, _INT, _P]),
     "fl_rotate_adjoint": (_INT, [_P, _P, _I64, _I64, _I64, _F, _F, _INT, _P]),
     "fl_mlem_multi_workspace_bytes": (_SZ, [_P, _I64]),
+    "fl_mlem_multi_workspace_bytes_sm": (_SZ, [_P, _I64]),
     "fl_mlem_multi_create": (_INT, [_P, _I64, _D, _D, _P, _SZ, _P, ctypes.POINTER(_P)]),
     "fl_mlem_multi_run": (_INT, [_P, _I64, _INT, _P]),
     "fl_mlem_multi_destroy": (_INT, [_P]),

@@ -28,6 +28,7 @@ from . import _lib
 
 FWD_ANGLES = (8, 4)
 BWD_ROWS = (31, 15)
+SLICE_MAJOR_HEADROOM = 2 << 30  # bytes of device memory left free by the 'auto' layout
 
 
 @dataclass
@@ -77,9 +78,17 @@ class BatchMLEM:
     shapes: None (library defaults), a list of (fwd_angles, bwd_rows) per
     problem, or an ``Autotuner``.  beta > 0 selects the One-Step-Late Huber
     update (atten_cuda.cpp:330-341), as in ``mlem_cuda_batch``.
+
+    layout: 'slice_major' packs the problems that share a theta tensor and
+    (A, H, W) -- the emission lines of one sample -- along the slice axis and
+    back-projects each pack from a slice-major copy of its attenuation (the
+    workspace then holds that copy: about the attenuation's size again);
+    'reference' back-projects every problem from its own attenuation; 'auto'
+    takes 'slice_major' when its workspace fits the free device memory with
+    2 GiB to spare.
     """
 
-    def __init__(self, problems, beta=0.0, delta=0.01, shapes=None):
+    def __init__(self, problems, beta=0.0, delta=0.01, shapes=None, layout="auto"):
         if not problems:
             raise ValueError("empty batch")
         for i, p in enumerate(problems):
@@ -95,9 +104,21 @@ class BatchMLEM:
         self.problems = list(problems)  # keeps every array alive while the plan uses it
         self._arr = _structs(self.problems, self.shapes)
         lib = _lib.load()
-        nbytes = int(lib.fl_mlem_multi_workspace_bytes(ctypes.cast(self._arr, ctypes.c_void_p), len(problems)))
+        arr = ctypes.cast(self._arr, ctypes.c_void_p)
+        nbytes = int(lib.fl_mlem_multi_workspace_bytes(arr, len(problems)))
         if nbytes == 0:
             raise ValueError("invalid batch")
+        if layout not in ("auto", "slice_major", "reference"):
+            raise ValueError(f"layout must be 'auto', 'slice_major' or 'reference', not {layout!r}")
+        sm_bytes = int(lib.fl_mlem_multi_workspace_bytes_sm(arr, len(problems)))
+        if layout == "auto":
+            free, _ = torch.cuda.mem_get_info(self.device)
+            layout = "slice_major" if 0 < sm_bytes and sm_bytes + SLICE_MAJOR_HEADROOM <= free else "reference"
+        if layout == "slice_major":
+            if sm_bytes == 0:
+                raise RuntimeError("the slice-major layout does not apply: a pack has H*W*sum(B) >= 2^29")
+            nbytes = sm_bytes
+        self.layout = layout
         # 256-byte aligned workspace inside a slightly larger buffer
         self._ws_buf = torch.empty(nbytes + 256, dtype=torch.uint8, device=self.device)
         off = (-self._ws_buf.data_ptr()) % 256

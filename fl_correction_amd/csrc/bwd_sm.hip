@@ -45,6 +45,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <vector>
 
 namespace fl {
 
@@ -228,8 +229,9 @@ struct SmEdgeCand {
     float w00, w01, w10, w11;
 };
 
+// Workgroup `id` of one problem's edge launch.
 template <int SPL, bool RATIO>
-__global__ __launch_bounds__(64 * kSmEW) void sm_edge_kernel(SmArgs p)
+__device__ __forceinline__ void sm_edge_body(const SmArgs &p, uint32_t id)
 {
     constexpr int S = 64 * SPL;
     constexpr int RING = 8;
@@ -237,8 +239,8 @@ __global__ __launch_bounds__(64 * kSmEW) void sm_edge_kernel(SmArgs p)
     const SmGeom &g = p.g;
     // items in dispatch order (the dispatcher deals consecutive workgroups to
     // the XCDs in turn), slice groups innermost
-    const int grp = (int)(blockIdx.x % (uint32_t)g.Gi);
-    const int k = (int)(blockIdx.x / (uint32_t)g.Gi);
+    const int grp = (int)(id % (uint32_t)g.Gi);
+    const int k = (int)(id / (uint32_t)g.Gi);
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const int b0 = grp * S + lane * SPL;  // this lane's first slice (B is a multiple of SPL)
     const float xmax = g.H - 1.001f, ymax = g.W - 1.001f;
@@ -442,14 +444,20 @@ __global__ __launch_bounds__(64 * kSmEW) void sm_edge_kernel(SmArgs p)
     for (int q = wave; q < kSmEdgeVox; q += kSmEW) e[q * 64 + lane] = win[q * 64 + lane];
 }
 
+template <int SPL, bool RATIO>
+__global__ __launch_bounds__(64 * kSmEW) void sm_edge_kernel(SmArgs p)
+{
+    sm_edge_body<SPL, RATIO>(p, blockIdx.x);
+}
+
 // Sum of a strip's windows over its depth bins and angle ranges, in that fixed
 // order: one workgroup per (slice group, side, strip, window entry).
 template <int SPL>
-__global__ __launch_bounds__(64) void sm_edge_reduce_kernel(SmArgs p)
+__device__ __forceinline__ void sm_edge_reduce_body(const SmArgs &p, uint32_t wg)
 {
     const SmGeom &g = p.g;
     const int ns = g.nsi > g.nsj ? g.nsi : g.nsj;
-    int id = blockIdx.x;
+    int id = (int)wg;
     const int q = id % kSmEdgeVox;
     id /= kSmEdgeVox;
     const int t = id % ns;
@@ -468,6 +476,12 @@ __global__ __launch_bounds__(64) void sm_edge_reduce_kernel(SmArgs p)
     }
     reinterpret_cast<SmVec<SPL> *>(p.edge_sum)[((((int64_t)grp * 4 + side) * ns + t) * kSmEdgeVox + q) * 64 + lane] =
         acc;
+}
+
+template <int SPL>
+__global__ __launch_bounds__(64) void sm_edge_reduce_kernel(SmArgs p)
+{
+    sm_edge_reduce_body<SPL>(p, blockIdx.x);
 }
 
 // ---------------------------------------------------------------------------
@@ -516,15 +530,15 @@ struct SmCand {
     float w00, w01, w10, w11;
 };
 
+// Workgroup `id` of one problem's interior launch.
 template <int SPL, bool RATIO>
-__global__ __launch_bounds__(64) void sm_interior_kernel(SmArgs p)
+__device__ __forceinline__ void sm_interior_body(const SmArgs &p, uint32_t id)
 {
     constexpr int S = 64 * SPL;
     // hits whose loads are in flight
     constexpr int RING = 8;
     const SmGeom &g = p.g;
     const int nint = g.nbi * g.nbj;
-    const uint32_t id = xcd_remap(blockIdx.x, gridDim.x);
     const int grp = (int)(id / (uint32_t)nint);
     const int k = (int)(id % (uint32_t)nint);
     const int bi = k / g.nbj, bj = k % g.nbj;
@@ -658,6 +672,12 @@ __global__ __launch_bounds__(64) void sm_interior_kernel(SmArgs p)
     for (int yy = 0; yy < kSmTB; ++yy) hal[(kSmWB + yy) * 64 + lane] = acc[yy * kSmWB + kSmTB];
 }
 
+template <int SPL, bool RATIO>
+__global__ __launch_bounds__(64) void sm_interior_kernel(SmArgs p)
+{
+    sm_interior_body<SPL, RATIO>(p, xcd_remap(blockIdx.x, gridDim.x));
+}
+
 // ---------------------------------------------------------------------------
 // Finalize: per voxel the fixed-order sum of every item's contribution, back to
 // (B, H, W), with the output mode applied.  One workgroup: one image row y,
@@ -710,11 +730,12 @@ __device__ __forceinline__ float sm_voxel_sum(const SmFin &f, int b, int y, int 
     return v;
 }
 
-__global__ __launch_bounds__(64) void sm_finalize_kernel(SmFin f)
+// Workgroup (columns 64*bx.., row y, slices 64*c64..) of one problem's finalize.
+__device__ __forceinline__ void sm_finalize_body(const SmFin &f, int bx, int y, int c64)
 {
     __shared__ float tile[64 * 65];
     const SmGeom &g = f.g;
-    const int x0 = blockIdx.x * 64, y = blockIdx.y, c64 = blockIdx.z;
+    const int x0 = bx * 64;
     const int lane = threadIdx.x;
     // phase 1: lane = slice; voxels x0..x0+63 of row y
     const int nx = min(64, g.W - x0);
@@ -748,17 +769,64 @@ __global__ __launch_bounds__(64) void sm_finalize_kernel(SmFin f)
     }
 }
 
+__global__ __launch_bounds__(64) void sm_finalize_kernel(SmFin f)
+{
+    sm_finalize_body(f, blockIdx.x, blockIdx.y, blockIdx.z);
+}
+
+// ---------------------------------------------------------------------------
+// Grouped launches over many problems (batched MLEM): job j of a launch owns
+// the workgroups [begin[j], begin[j+1]) and runs them exactly as its own
+// launch would.  One launch per kernel and slices-per-lane class.
+// ---------------------------------------------------------------------------
+template <int SPL>
+__global__ __launch_bounds__(64) void sm_interior_group_kernel(const SmArgs *jobs, const uint32_t *begin, int n)
+{
+    const uint32_t t = xcd_remap(blockIdx.x, gridDim.x);
+    const int j = find_job(begin, n, t);
+    const SmArgs p = scalar_record(jobs + j);
+    sm_interior_body<SPL, true>(p, t - scalar_read_u32(begin, j));
+}
+
+template <int SPL>
+__global__ __launch_bounds__(64 * kSmEW) void sm_edge_group_kernel(const SmArgs *jobs, const uint32_t *begin, int n)
+{
+    const int j = find_job(begin, n, blockIdx.x);
+    const SmArgs p = scalar_record(jobs + j);
+    sm_edge_body<SPL, true>(p, blockIdx.x - scalar_read_u32(begin, j));
+}
+
+template <int SPL>
+__global__ __launch_bounds__(64) void sm_edge_reduce_group_kernel(const SmArgs *jobs, const uint32_t *begin, int n)
+{
+    const int j = find_job(begin, n, blockIdx.x);
+    const SmArgs p = scalar_record(jobs + j);
+    sm_edge_reduce_body<SPL>(p, blockIdx.x - scalar_read_u32(begin, j));
+}
+
+__global__ __launch_bounds__(64) void sm_finalize_group_kernel(const SmFin *jobs, const uint32_t *begin, int n)
+{
+    const int j = find_job(begin, n, blockIdx.x);
+    const SmFin f = scalar_record(jobs + j);
+    uint32_t id = blockIdx.x - scalar_read_u32(begin, j);
+    const uint32_t nbx = (uint32_t)(f.g.W + 63) / 64;
+    const int bx = (int)(id % nbx);
+    id /= nbx;
+    sm_finalize_body(f, bx, (int)(id % (uint32_t)f.g.H), (int)(id / (uint32_t)f.g.H));
+}
+
 // ---------------------------------------------------------------------------
 // (N, B, L) -> (N, L, B): the slice-major copy.  64 x 64 tiles through LDS.
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(256) void sm_transpose_kernel(const float *src, float *dst, int B, int64_t L)
+// dst rows hold Bp >= B slices (a pitched copy writes slices 0..B-1 of them).
+__global__ __launch_bounds__(256) void sm_transpose_kernel(const float *src, float *dst, int B, int64_t L, int Bp)
 {
     __shared__ float tile[64 * 65];
     const int64_t l0 = (int64_t)blockIdx.x * 64;
     const int b0 = blockIdx.y * 64;
     const int64_t n = blockIdx.z;
     const float *s = src + n * (int64_t)B * L;
-    float *d = dst + n * (int64_t)B * L;
+    float *d = dst + n * (int64_t)Bp * L;
     const int tx = threadIdx.x & 63, ty = threadIdx.x >> 6;
     for (int r = ty; r < 64; r += 4) {
         const int bb = b0 + r;
@@ -769,8 +837,17 @@ __global__ __launch_bounds__(256) void sm_transpose_kernel(const float *src, flo
     for (int r = ty; r < 64; r += 4) {
         const int64_t l = l0 + r;
         const int bb = b0 + tx;
-        if (l < L && bb < B) d[l * B + bb] = tile[tx * 65 + r];
+        if (l < L && bb < B) d[l * Bp + bb] = tile[tx * 65 + r];
     }
+}
+
+// ratio[(a * W + w) * Bp + b] = em[a], b < B: one workgroup per (w, a).
+__global__ __launch_bounds__(64) void sm_fill_em_kernel(float *ratio, const float *em, int W, int B, int Bp)
+{
+    const int w = blockIdx.x, a = blockIdx.y;
+    const float e = em[a];
+    float *r = ratio + ((int64_t)a * W + w) * Bp;
+    for (int b = threadIdx.x; b < B; b += 64) r[b] = e;
 }
 
 // ---------------------------------------------------------------------------
@@ -820,15 +897,74 @@ size_t sm_scratch_bytes(int64_t A, int64_t B, int64_t H, int64_t W)
     return sm_align(n[0]) + sm_align(n[1]) + sm_align(n[2]) + sm_align(n[3]);
 }
 
-int launch_sm_transpose(const float *src, float *dst, int64_t N, int64_t B, int64_t L, hipStream_t s)
+int launch_sm_transpose_pitched(const float *src, float *dst, int64_t N, int64_t B, int64_t L, int64_t Bp,
+                                hipStream_t s)
 {
     if (N == 0 || B == 0 || L == 0) return FL_OK;
-    FL_REQUIRE(N < 65536 && (L + 63) / 64 < (int64_t)UINT32_MAX, "slice-major copy: shape too large");
+    FL_REQUIRE(N < 65536 && B <= Bp && Bp < (1 << 24) && (L + 63) / 64 < (int64_t)UINT32_MAX,
+               "slice-major copy: shape too large");
     sm_transpose_kernel<<<dim3((uint32_t)((L + 63) / 64), (uint32_t)((B + 63) / 64), (uint32_t)N), dim3(256), 0,
-                          s>>>(src, dst, (int)B, L);
+                          s>>>(src, dst, (int)B, L, (int)Bp);
     FL_LAUNCH_CHECK();
     return FL_OK;
 }
+
+int launch_sm_transpose(const float *src, float *dst, int64_t N, int64_t B, int64_t L, hipStream_t s)
+{
+    return launch_sm_transpose_pitched(src, dst, N, B, L, B, s);
+}
+
+int launch_sm_fill_em(float *ratio, const float *em, int64_t A, int64_t W, int64_t B, int64_t Bp, hipStream_t s)
+{
+    if (A == 0 || W == 0 || B == 0) return FL_OK;
+    FL_REQUIRE(A < 65536 && W < ((int64_t)1 << 31) && B <= Bp && Bp < (1 << 24), "em fill: shape too large");
+    sm_fill_em_kernel<<<dim3((uint32_t)W, (uint32_t)A), dim3(64), 0, s>>>(ratio, em, (int)W, (int)B, (int)Bp);
+    FL_LAUNCH_CHECK();
+    return FL_OK;
+}
+
+// Launch records of one problem: the geometry and the scratch regions.
+static SmArgs sm_args(const SmGeom &g, const float *atten_sm, const float *ratio, const float *em,
+                      const float *theta, void *scratch)
+{
+    size_t n[4];
+    sm_sizes(g, n);
+    char *ws = (char *)scratch;
+    SmArgs p{};
+    p.g = g;
+    p.atten = atten_sm;
+    p.ratio = ratio;
+    p.em = em;
+    p.theta = theta;
+    p.own = (float *)ws;
+    p.halo = (float *)(ws + sm_align(n[0]));
+    p.edge = (float *)(ws + sm_align(n[0]) + sm_align(n[1]));
+    p.edge_sum = (float *)(ws + sm_align(n[0]) + sm_align(n[1]) + sm_align(n[2]));
+    return p;
+}
+
+static SmFin sm_fin(const SmArgs &p, float *out, int mode, const float *C, const float *sens, float beta,
+                    float delta)
+{
+    SmFin f{};
+    f.g = p.g;
+    f.own = p.own;
+    f.halo = p.halo;
+    f.edge_sum = p.edge_sum;
+    f.out = out;
+    f.mode = mode;
+    f.C = C;
+    f.sens = sens;
+    f.beta = beta;
+    f.delta = delta;
+    return f;
+}
+
+// Workgroups of each launch of one problem.
+static int64_t sm_interior_wgs(const SmGeom &g) { return (int64_t)g.Gi * g.nbi * g.nbj; }
+static int64_t sm_edge_wgs(const SmGeom &g) { return (int64_t)g.Gi * sm_edge_items(g); }
+static int64_t sm_reduce_wgs(const SmGeom &g) { return (int64_t)g.Gi * 4 * std::max(g.nsi, g.nsj) * kSmEdgeVox; }
+static int64_t sm_finalize_wgs(const SmGeom &g) { return (int64_t)((g.W + 63) / 64) * g.H * ((g.B + 63) / 64); }
 
 template <int SPL>
 static void sm_launch(const SmArgs &p, uint32_t n_int, uint32_t n_edge, uint32_t n_red, hipStream_t stream)
@@ -851,22 +987,10 @@ int launch_sm_backward(const float *atten_sm, const float *ratio_sm, const float
     const SmGeom g = sm_geometry(A, B, H, W);
     FL_REQUIRE(scratch_bytes >= sm_scratch_bytes(A, B, H, W) && scratch, "slice-major back-projector: scratch too small");
     FL_REQUIRE(H * W * B < ((int64_t)1 << 29), "slice-major back-projector: H*W*B must be below 2^29");
-    size_t n[4];
-    sm_sizes(g, n);
-    char *ws = (char *)scratch;
-    SmArgs p{};
-    p.g = g;
-    p.atten = atten_sm;
-    p.ratio = ratio_sm;
-    p.em = em;
-    p.theta = theta;
-    p.own = (float *)ws;
-    p.halo = (float *)(ws + sm_align(n[0]));
-    p.edge = (float *)(ws + sm_align(n[0]) + sm_align(n[1]));
-    p.edge_sum = (float *)(ws + sm_align(n[0]) + sm_align(n[1]) + sm_align(n[2]));
-    const int64_t n_int = (int64_t)g.Gi * g.nbi * g.nbj;
-    const int64_t n_e = (int64_t)g.Gi * sm_edge_items(g);
-    const int64_t n_red = (int64_t)g.Gi * 4 * std::max(g.nsi, g.nsj) * kSmEdgeVox;
+    const SmArgs p = sm_args(g, atten_sm, ratio_sm, em, theta, scratch);
+    const int64_t n_int = sm_interior_wgs(g);
+    const int64_t n_e = sm_edge_wgs(g);
+    const int64_t n_red = sm_reduce_wgs(g);
     FL_REQUIRE(n_int < (int64_t)UINT32_MAX && n_e < (int64_t)UINT32_MAX && n_red < (int64_t)UINT32_MAX,
                "slice-major back-projector: grid too large");
     if (g.spl == 4)
@@ -876,19 +1000,143 @@ int launch_sm_backward(const float *atten_sm, const float *ratio_sm, const float
     else
         sm_launch<1>(p, (uint32_t)n_int, (uint32_t)n_e, (uint32_t)n_red, stream);
     FL_LAUNCH_CHECK();
-    SmFin f{};
-    f.g = g;
-    f.own = p.own;
-    f.halo = p.halo;
-    f.edge_sum = p.edge_sum;
-    f.out = out;
-    f.mode = mode;
-    f.C = C;
-    f.sens = sens;
-    f.beta = beta;
-    f.delta = delta;
+    const SmFin f = sm_fin(p, out, mode, C, sens, beta, delta);
     sm_finalize_kernel<<<dim3((uint32_t)((W + 63) / 64), (uint32_t)H, (uint32_t)((B + 63) / 64)), dim3(64), 0,
                          stream>>>(f);
+    FL_LAUNCH_CHECK();
+    return FL_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Grouped launches (batched MLEM)
+// ---------------------------------------------------------------------------
+static size_t tb_align(size_t b) { return (b + 255) / 256 * 256; }
+
+// Table image: SmArgs[n] (grouped by slices-per-lane class), three begin arrays
+// per class, then n_var SmFin[n] and the finalize begin array.
+size_t sm_group_table_bytes(int n)
+{
+    if (n < 1) return 0;
+    size_t b = tb_align(n * sizeof(SmArgs));
+    b += 3 * 3 * tb_align((size_t)(n + 1) * sizeof(uint32_t));
+    b += kSmGroupVariants * tb_align(n * sizeof(SmFin));
+    b += tb_align((size_t)(n + 1) * sizeof(uint32_t));
+    return b;
+}
+
+int sm_group_build(const SmPack *packs, int n, const SmFinSpec *fin, int n_var, char *host, SmGroup *out)
+{
+    FL_REQUIRE(n >= 1 && n_var >= 1 && n_var <= kSmGroupVariants && packs && fin && host && out,
+               "slice-major group: bad arguments");
+    std::vector<SmGeom> geo(n);
+    for (int i = 0; i < n; ++i) {
+        const SmPack &q = packs[i];
+        FL_REQUIRE(q.A >= 1 && q.B >= 1 && q.H >= 2 && q.W >= 2 && q.A < 65536 && q.H * q.W * q.B < ((int64_t)1 << 29),
+                   "slice-major group: a problem's shape is out of range (H*W*B must be below 2^29)");
+        FL_REQUIRE(q.atten_sm && q.ratio && q.em && q.theta && q.scratch, "slice-major group: null pointer");
+        geo[i] = sm_geometry(q.A, q.B, q.H, q.W);
+    }
+    SmGroup G;
+    G.n = n;
+    G.n_var = n_var;
+    size_t o = tb_align(n * sizeof(SmArgs));
+    const size_t begin_bytes = tb_align((size_t)(n + 1) * sizeof(uint32_t));
+    SmArgs *jobs = (SmArgs *)host;
+    int k = 0;
+    const int spls[3] = {4, 2, 1};
+    for (int c = 0; c < 3; ++c) {
+        // this class's problems, most angles first (their workgroups run longest)
+        std::vector<int> idx;
+        for (int i = 0; i < n; ++i)
+            if (geo[i].spl == spls[c]) idx.push_back(i);
+        std::stable_sort(idx.begin(), idx.end(), [&](int a, int b) { return geo[a].A > geo[b].A; });
+        SmGroup::Class &cl = G.cls[c];
+        cl.spl = spls[c];
+        cl.n = (int)idx.size();
+        cl.jobs = (size_t)k * sizeof(SmArgs);
+        cl.b_int = o;
+        cl.b_edge = o + begin_bytes;
+        cl.b_red = o + 2 * begin_bytes;
+        uint32_t *bi = (uint32_t *)(host + cl.b_int), *be = (uint32_t *)(host + cl.b_edge),
+                 *br = (uint32_t *)(host + cl.b_red);
+        int64_t ti = 0, te = 0, tr = 0;
+        for (size_t m = 0; m < idx.size(); ++m) {
+            const SmPack &q = packs[idx[m]];
+            jobs[k + m] = sm_args(geo[idx[m]], q.atten_sm, q.ratio, q.em, q.theta, q.scratch);
+            bi[m] = (uint32_t)ti;
+            be[m] = (uint32_t)te;
+            br[m] = (uint32_t)tr;
+            ti += sm_interior_wgs(geo[idx[m]]);
+            te += sm_edge_wgs(geo[idx[m]]);
+            tr += sm_reduce_wgs(geo[idx[m]]);
+        }
+        // workgroups x threads must stay below 2^32
+        FL_REQUIRE(ti < ((int64_t)1 << 26) && te < ((int64_t)1 << 24) && tr < ((int64_t)1 << 26),
+                   "slice-major group: too many workgroups for one launch");
+        bi[idx.size()] = (uint32_t)ti;
+        be[idx.size()] = (uint32_t)te;
+        br[idx.size()] = (uint32_t)tr;
+        cl.t_int = (uint32_t)ti;
+        cl.t_edge = (uint32_t)te;
+        cl.t_red = (uint32_t)tr;
+        k += (int)idx.size();
+        o += 3 * begin_bytes;
+    }
+    // finalize variants in pack order
+    for (int v = 0; v < n_var; ++v) {
+        G.fin[v] = o;
+        SmFin *fv = (SmFin *)(host + o);
+        for (int i = 0; i < n; ++i) {
+            const SmPack &q = packs[i];
+            const SmFinSpec &f = fin[(size_t)v * n + i];
+            FL_REQUIRE(f.out != nullptr, "slice-major group: null output");
+            const SmArgs p = sm_args(geo[i], q.atten_sm, q.ratio, q.em, q.theta, q.scratch);
+            fv[i] = sm_fin(p, f.out, f.mode, f.C, f.sens, f.beta, f.delta);
+        }
+        o += tb_align(n * sizeof(SmFin));
+    }
+    o += (size_t)(kSmGroupVariants - n_var) * tb_align(n * sizeof(SmFin));
+    G.b_fin = o;
+    uint32_t *bf = (uint32_t *)(host + o);
+    int64_t tf = 0;
+    for (int i = 0; i < n; ++i) {
+        bf[i] = (uint32_t)tf;
+        tf += sm_finalize_wgs(geo[i]);
+    }
+    FL_REQUIRE(tf < ((int64_t)1 << 26), "slice-major group: too many workgroups for one launch");
+    bf[n] = (uint32_t)tf;
+    G.t_fin = (uint32_t)tf;
+    *out = G;
+    return FL_OK;
+}
+
+template <int SPL>
+static void sm_group_class(const SmGroup &G, const SmGroup::Class &c, hipStream_t s)
+{
+    const SmArgs *jobs = (const SmArgs *)(G.dev + c.jobs);
+    const uint32_t *bi = (const uint32_t *)(G.dev + c.b_int), *be = (const uint32_t *)(G.dev + c.b_edge),
+                   *br = (const uint32_t *)(G.dev + c.b_red);
+    if (c.t_int) sm_interior_group_kernel<SPL><<<dim3(c.t_int), dim3(64), 0, s>>>(jobs, bi, c.n);
+    if (c.t_edge) sm_edge_group_kernel<SPL><<<dim3(c.t_edge), dim3(64 * kSmEW), 0, s>>>(jobs, be, c.n);
+    if (c.t_red) sm_edge_reduce_group_kernel<SPL><<<dim3(c.t_red), dim3(64), 0, s>>>(jobs, br, c.n);
+}
+
+int launch_sm_group(const SmGroup &G, int variant, hipStream_t s)
+{
+    FL_REQUIRE(G.dev != nullptr && variant >= 0 && variant < G.n_var, "slice-major group: bad variant");
+    for (int c = 0; c < 3; ++c) {
+        const SmGroup::Class &cl = G.cls[c];
+        if (cl.n == 0) continue;
+        if (cl.spl == 4)
+            sm_group_class<4>(G, cl, s);
+        else if (cl.spl == 2)
+            sm_group_class<2>(G, cl, s);
+        else
+            sm_group_class<1>(G, cl, s);
+        FL_LAUNCH_CHECK();
+    }
+    sm_finalize_group_kernel<<<dim3(G.t_fin), dim3(64), 0, s>>>((const SmFin *)(G.dev + G.fin[variant]),
+                                                                 (const uint32_t *)(G.dev + G.b_fin), G.n);
     FL_LAUNCH_CHECK();
     return FL_OK;
 }

@@ -251,13 +251,15 @@ __device__ __forceinline__ void fwd_strip_tile(const FwdArgs &p, uint32_t t)
         float tot = ((rk[0] + rk[64]) + rk[128]) + rk[192];
         if (p.nref == 1) tot *= emv[k * kFRefChunk];
         // (A, B, W), or slice-major (A, W, B) for the slice-major back-projector
-        const int64_t o = p.out_sm ? ((int64_t)(a0 + k) * p.W + wo) * p.B + b : ((int64_t)(a0 + k) * p.B + b) * p.W + wo;
+        const int64_t o = p.out_sm ? ((int64_t)(a0 + k) * p.W + wo) * (p.out_pitch ? p.out_pitch : p.B) + b
+                                   : ((int64_t)(a0 + k) * p.B + b) * p.W + wo;
         const int64_t oi = ((int64_t)(a0 + k) * p.B + b) * p.W + wo;
         if (p.mode == kFwdAccumulate || p.mode == kFwdAccumulateRatio) tot += p.out[o];
         if (p.mode >= kFwdRatio) {
             // Pf.clamp_min_(1e-6); ratio = I / Pf   (atten_cuda.cpp:326-328)
             const float pf = tot < 1e-6f ? 1e-6f : tot;
             tot = p.I[oi] / pf;
+            if (p.fold_em) tot *= emv[k * kFRefChunk];
         }
         p.out[o] = tot;
     }

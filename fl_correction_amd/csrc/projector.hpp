@@ -46,6 +46,8 @@ struct FwdArgs {
     int ntw, n_groups;
     int group_angles;  // grouped launches: angles per workgroup of this job (4 or 8)
     int out_sm;        // write out as (A, W, B) instead of (A, B, W)
+    int out_pitch;     // slice-major output: slices per (a, w) row of out (0: B)
+    int fold_em;       // ratio modes: write em[a] * ratio (the back-projector then takes em = 1)
 };
 
 // ---------------------------------------------------------------------------
@@ -151,5 +153,45 @@ int launch_sm_backward(const float *atten_sm, const float *ratio_sm, const float
                        hipStream_t stream);
 // (N, B, L) -> (N, L, B)
 int launch_sm_transpose(const float *src, float *dst, int64_t N, int64_t B, int64_t L, hipStream_t s);
+// (N, B, L) -> slices 0..B-1 of (N, L, Bp), dst pointing at the first slice
+int launch_sm_transpose_pitched(const float *src, float *dst, int64_t N, int64_t B, int64_t L, int64_t Bp,
+                                hipStream_t s);
+// ratio[(a * W + w) * Bp + b] = em[a] for b < B (the back-projection of em: the sensitivity)
+int launch_sm_fill_em(float *ratio, const float *em, int64_t A, int64_t W, int64_t B, int64_t Bp, hipStream_t s);
+
+// Grouped slice-major back-projection over many problems (batched MLEM): every
+// problem ("pack") has its own atten_sm, ratio (A, W, B), em, theta and scratch;
+// the tables are built on the host once and replayed.  Ratio is always given.
+struct SmArgs;
+struct SmFin;
+struct SmPack {
+    const float *atten_sm, *ratio, *em, *theta;
+    int64_t A, B, H, W;
+    void *scratch;  // sm_scratch_bytes(A, B, H, W)
+};
+struct SmFinSpec {  // one finalize variant of one pack
+    float *out;
+    int mode;  // BwdMode
+    const float *C, *sens;
+    float beta, delta;
+};
+constexpr int kSmGroupVariants = 3;
+struct SmGroup {
+    struct Class {
+        int spl = 0, n = 0;
+        size_t jobs = 0, b_int = 0, b_edge = 0, b_red = 0;  // byte offsets in the table image
+        uint32_t t_int = 0, t_edge = 0, t_red = 0;           // workgroups per launch
+    } cls[3];
+    int n = 0, n_var = 0;
+    size_t fin[kSmGroupVariants] = {}, b_fin = 0;
+    uint32_t t_fin = 0;
+    const char *dev = nullptr;  // device address of the table image
+};
+size_t sm_group_table_bytes(int n);
+// Writes the tables of n packs into host[0, sm_group_table_bytes(n)); fin holds
+// n_var finalize variants, fin[v * n + i] for pack i.  out->dev is set by the caller.
+int sm_group_build(const SmPack *packs, int n, const SmFinSpec *fin, int n_var, char *host, SmGroup *out);
+// Back-projection of every pack, then finalize variant `variant`.
+int launch_sm_group(const SmGroup &g, int variant, hipStream_t s);
 
 }  // namespace fl

@@ -208,10 +208,24 @@ typedef struct fl_mlem_multi fl_mlem_multi;
  * Host-only arithmetic. */
 FL_API size_t fl_mlem_multi_workspace_bytes(const fl_mlem_problem *problems, int64_t n);
 
+/* Workspace of the slice-major mode, which fl_mlem_multi_create takes when its
+ * workspace is at least this large: problems with the same theta array (the
+ * same pointer) and the same A, H, W -- the emission lines of one sample -- are
+ * packed along the slice axis into one problem of sum(B) slices, and per pack
+ * the workspace holds a slice-major copy of the attenuation (A, H, W, sum B),
+ * the ratio (A, W, sum B), the sensitivity, two estimates and the
+ * back-projector's scratch.  Slices are independent, so the results equal
+ * each problem's own to rounding.  0 when the mode does not apply (a pack
+ * with H*W*sum(B) >= 2^29 or A >= 65536) or for invalid arguments.  Host-only
+ * arithmetic.  No counterpart in the reference, which runs one problem per call. */
+FL_API size_t fl_mlem_multi_workspace_bytes_sm(const fl_mlem_problem *problems, int64_t n);
+
 /* Validate the problems (shapes, tile shapes, disjoint C), build and upload the
  * job tables into `workspace` (device memory, 256-byte aligned, of at least
- * fl_mlem_multi_workspace_bytes bytes) and enqueue the sensitivity pass
- * sens = max(B^T 1, 1e-6) on `stream`.  The problem arrays (not the struct
+ * fl_mlem_multi_workspace_bytes bytes; with at least
+ * fl_mlem_multi_workspace_bytes_sm bytes the slice-major mode) and enqueue the
+ * sensitivity pass sens = max(B^T 1, 1e-6) on `stream` (in the slice-major
+ * mode after the slice-major copies).  The problem arrays (not the struct
  * array) must stay valid while the plan is used.  Allocates no device memory. */
 FL_API int fl_mlem_multi_create(const fl_mlem_problem *problems, int64_t n, double beta,
                                 double delta, void *workspace, size_t workspace_bytes,

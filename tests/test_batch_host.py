@@ -42,6 +42,40 @@ def test_workspace_bytes():
     assert lib.fl_mlem_multi_workspace_bytes(ctypes.cast(arr, ctypes.c_void_p), 0) == 0
 
 
+def test_workspace_bytes_slice_major_packs():
+    """Problems with the same theta pointer and (A, H, W) share one pack: its
+    slice-major attenuation, ratio, sensitivity, two estimates and scratch are
+    sized for the summed slice count; a pack past H*W*sum(B) = 2^29 makes the
+    mode inapplicable (0)."""
+    lib = _lib_or_skip()
+    al = lambda b: (b + 255) // 256 * 256
+
+    def sizes(shapes, thetas):
+        arr = _arr(shapes)
+        for i, t in enumerate(thetas):
+            arr[i].theta = t
+        p = ctypes.cast(arr, ctypes.c_void_p)
+        return lib.fl_mlem_multi_workspace_bytes(p, len(shapes)), lib.fl_mlem_multi_workspace_bytes_sm(p, len(shapes))
+
+    def pack_bytes(A, B, H, W):
+        return (al(4 * A * H * W * B) + al(4 * A * W * B) + 3 * al(4 * B * H * W)
+                + al(lib.fl_backproject_sm_scratch_bytes(A, B, H, W)))
+
+    shapes = [(30, 3, 64, 64), (30, 5, 64, 64), (12, 2, 40, 70)]
+    dense, packed = sizes(shapes, [8, 8, 16])  # lines 0 and 1 share theta
+    _, apart = sizes(shapes, [8, 24, 16])
+    per = pack_bytes(30, 8, 64, 64) + pack_bytes(12, 2, 40, 70)
+    assert packed % 256 == 0 and packed > dense
+    assert per < packed < per + 64 * 1024  # plus the tables
+    per_apart = pack_bytes(30, 3, 64, 64) + pack_bytes(30, 5, 64, 64) + pack_bytes(12, 2, 40, 70)
+    assert per_apart < apart < per_apart + 64 * 1024
+    # two lines of 2^28 voxels each: fine apart, too large packed
+    big = [(1, 4096, 256, 256), (1, 4096, 256, 256)]
+    assert sizes(big, [8, 16])[1] > 0
+    assert sizes(big, [8, 8])[1] == 0
+    assert lib.fl_mlem_multi_workspace_bytes_sm(None, 3) == 0
+
+
 def test_create_rejects_before_device_work():
     lib = _lib_or_skip()
     plan = ctypes.c_void_p()
