@@ -68,11 +68,31 @@ int bwd_rows_of(const fl_mlem_problem &p) { return p.bwd_rows == 0 ? kBI : p.bwd
 
 // Problems packed along the slice axis (slice-major mode): same theta array and
 // the same A, H, W; members in input order, member m at slice offset b0[m].
+// The pack's slices are padded to Bp, a multiple of its slices per lane.
 struct Pack {
     std::vector<int64_t> members, b0;
-    int64_t A = 0, H = 0, W = 0, B = 0;
+    int64_t A = 0, H = 0, W = 0, B = 0, Bp = 0;
+    int spl = 1;
     const float *theta = nullptr;
 };
+
+// Slices per lane of a pack of B slices: the fewest wavefronts per block
+// (64 * spl slices each), then the fewest slices per lane (fewer registers,
+// more wavefronts resident).  A wavefront's cost hardly depends on how many
+// of its lanes hold slices.
+int pack_spl(int64_t B)
+{
+    int best = 1;
+    int64_t groups = (B + 63) / 64;
+    for (int spl : {2, 4}) {
+        const int64_t g = (B + 64 * spl - 1) / (64 * spl);
+        if (g < groups) {
+            groups = g;
+            best = spl;
+        }
+    }
+    return best;
+}
 
 std::vector<Pack> make_packs(const fl_mlem_problem *pr, int64_t n)
 {
@@ -93,6 +113,10 @@ std::vector<Pack> make_packs(const fl_mlem_problem *pr, int64_t n)
         pk->members.push_back(i);
         pk->b0.push_back(pk->B);
         pk->B += q.B;
+    }
+    for (Pack &pk : packs) {
+        pk.spl = pack_spl(pk.B);
+        pk.Bp = (pk.B + pk.spl - 1) / pk.spl * pk.spl;
     }
     return packs;
 }
@@ -143,7 +167,7 @@ Layout layout(const fl_mlem_problem *pr, int64_t n, bool sm)
     int64_t max_a = 1;
     for (const Pack &pk : packs) {
         max_a = std::max(max_a, pk.A);
-        if (pk.A >= 65536 || pk.H * pk.W * pk.B >= ((int64_t)1 << 29)) L.ok = false;
+        if (pk.A >= 65536 || pk.H * pk.W * pk.Bp >= ((int64_t)1 << 29)) L.ok = false;
     }
     L.sm_tables = take(sm_group_table_bytes((int)packs.size()));
     L.pack_in = take(n * sizeof(CopyJob));
@@ -152,13 +176,13 @@ Layout layout(const fl_mlem_problem *pr, int64_t n, bool sm)
     L.ones = take((size_t)max_a * sizeof(float));
     L.tables = o;
     for (const Pack &pk : packs) {
-        const size_t nC = (size_t)(pk.B * pk.H * pk.W) * sizeof(float);
-        L.atten_sm.push_back(take((size_t)(pk.A * pk.H * pk.W * pk.B) * sizeof(float)));
-        L.ratio.push_back(take((size_t)(pk.A * pk.W * pk.B) * sizeof(float)));
+        const size_t nC = (size_t)(pk.Bp * pk.H * pk.W) * sizeof(float);
+        L.atten_sm.push_back(take((size_t)(pk.A * pk.H * pk.W * pk.Bp) * sizeof(float)));
+        L.ratio.push_back(take((size_t)(pk.A * pk.W * pk.Bp) * sizeof(float)));
         L.sens.push_back(take(nC));
         L.alt.push_back(take(nC));
         L.alt2.push_back(take(nC));
-        L.scratch.push_back(L.ok ? take(sm_scratch_bytes(pk.A, pk.B, pk.H, pk.W)) : 0);
+        L.scratch.push_back(L.ok ? take(sm_scratch_bytes(pk.A, pk.Bp, pk.H, pk.W, pk.spl)) : 0);
     }
     L.total = o;
     return L;
@@ -299,7 +323,7 @@ int fl_mlem_multi_create(const fl_mlem_problem *problems, int64_t n, double beta
         if (sm) {
             // em[a] * ratio into the pack's (A, W, sum B) ratio at this problem's slices
             f.out_sm = 1;
-            f.out_pitch = (int)packs[k].B;
+            f.out_pitch = (int)packs[k].Bp;
             f.fold_em = 1;
         }
         fe[i] = f;
@@ -417,7 +441,8 @@ int fl_mlem_multi_create(const fl_mlem_problem *problems, int64_t n, double beta
             const Pack &pk = packs[k];
             float *sens = (float *)(ws + L.sens[k]), *c0 = (float *)(ws + L.alt[k]), *c1 = (float *)(ws + L.alt2[k]);
             sp[k] = SmPack{(const float *)(ws + L.atten_sm[k]), (const float *)(ws + L.ratio[k]),
-                           (const float *)(ws + L.ones), pk.theta, pk.A, pk.B, pk.H, pk.W, ws + L.scratch[k]};
+                           (const float *)(ws + L.ones), pk.theta, pk.A, pk.Bp, pk.H, pk.W, pk.spl,
+                           ws + L.scratch[k]};
             fin[k] = SmFinSpec{sens, kBwdSens, nullptr, nullptr, (float)beta, (float)delta};
             fin[np + k] = SmFinSpec{c1, kBwdUpdate, c0, sens, (float)beta, (float)delta};
             fin[2 * np + k] = SmFinSpec{c0, kBwdUpdate, c1, sens, (float)beta, (float)delta};
@@ -452,12 +477,21 @@ int fl_mlem_multi_create(const fl_mlem_problem *problems, int64_t n, double beta
         // sens = max(B^T em, 1e-6) per slice   (atten_cuda.cpp:308-314)
         for (size_t k = 0; k < packs.size(); ++k) {
             const Pack &pk = packs[k];
+            if (pk.Bp > pk.B) {
+                // padding slices: zero attenuation, ratio and estimates, so that they stay 0
+                const size_t at = L.atten_sm[k], end = L.alt2[k] + (size_t)(pk.Bp * pk.H * pk.W) * sizeof(float);
+                const hipError_t me = hipMemsetAsync(ws + at, 0, end - at, s);
+                if (me != hipSuccess) {
+                    set_error(std::string("padding clear: ") + hipGetErrorString(me));
+                    return fail(FL_ERR_HIP);
+                }
+            }
             for (size_t j = 0; j < pk.members.size(); ++j) {
                 const fl_mlem_problem &q = problems[pk.members[j]];
                 if ((e = launch_sm_transpose_pitched(q.atten, (float *)(ws + L.atten_sm[k]) + pk.b0[j], q.A, q.B,
-                                                     q.H * q.W, pk.B, s)))
+                                                     q.H * q.W, pk.Bp, s)))
                     return fail(e);
-                if ((e = launch_sm_fill_em((float *)(ws + L.ratio[k]) + pk.b0[j], q.em_cs, q.A, q.W, q.B, pk.B, s)))
+                if ((e = launch_sm_fill_em((float *)(ws + L.ratio[k]) + pk.b0[j], q.em_cs, q.A, q.W, q.B, pk.Bp, s)))
                     return fail(e);
             }
         }

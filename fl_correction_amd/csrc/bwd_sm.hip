@@ -853,7 +853,7 @@ __global__ __launch_bounds__(64) void sm_fill_em_kernel(float *ratio, const floa
 // ---------------------------------------------------------------------------
 // Host side
 // ---------------------------------------------------------------------------
-SmGeom sm_geometry(int64_t A, int64_t B, int64_t H, int64_t W)
+static SmGeom sm_geometry(int64_t A, int64_t B, int64_t H, int64_t W, int spl = 0)
 {
     SmGeom g{};
     g.A = (int)A;
@@ -868,8 +868,8 @@ SmGeom sm_geometry(int64_t A, int64_t B, int64_t H, int64_t W)
     // deepest sample beyond the clamp box, in bins of 8 (the last bin is open)
     g.nd = (int)std::ceil((reach - std::min(cx, cy) + 2.0) / 8.0) + 1;
     g.nch = (int)std::max<int64_t>(1, std::min<int64_t>(8, A / 16));
-    // slices per lane: as many as the slices fill
-    g.spl = (B % 4 == 0 && B >= 192) ? 4 : (B % 2 == 0 && B >= 96) ? 2 : 1;
+    // slices per lane: as many as the slices fill (B must be a multiple of a given spl)
+    g.spl = spl > 0 ? spl : (B % 4 == 0 && B >= 192) ? 4 : (B % 2 == 0 && B >= 96) ? 2 : 1;
     g.S = 64 * g.spl;
     g.Gi = (int)((B + g.S - 1) / g.S);
     g.nbi = (int)((H - 1 + kSmTB - 1) / kSmTB);
@@ -889,9 +889,9 @@ static void sm_sizes(const SmGeom &g, size_t n[4])
 
 static size_t sm_align(size_t floats) { return (floats * sizeof(float) + 255) / 256 * 256; }
 
-size_t sm_scratch_bytes(int64_t A, int64_t B, int64_t H, int64_t W)
+size_t sm_scratch_bytes(int64_t A, int64_t B, int64_t H, int64_t W, int spl)
 {
-    const SmGeom g = sm_geometry(A, B, H, W);
+    const SmGeom g = sm_geometry(A, B, H, W, spl);
     size_t n[4];
     sm_sizes(g, n);
     return sm_align(n[0]) + sm_align(n[1]) + sm_align(n[2]) + sm_align(n[3]);
@@ -1034,7 +1034,9 @@ int sm_group_build(const SmPack *packs, int n, const SmFinSpec *fin, int n_var, 
         FL_REQUIRE(q.A >= 1 && q.B >= 1 && q.H >= 2 && q.W >= 2 && q.A < 65536 && q.H * q.W * q.B < ((int64_t)1 << 29),
                    "slice-major group: a problem's shape is out of range (H*W*B must be below 2^29)");
         FL_REQUIRE(q.atten_sm && q.ratio && q.em && q.theta && q.scratch, "slice-major group: null pointer");
-        geo[i] = sm_geometry(q.A, q.B, q.H, q.W);
+        FL_REQUIRE((q.spl == 1 || q.spl == 2 || q.spl == 4) && q.B % q.spl == 0,
+                   "slice-major group: slices per lane must be 1, 2 or 4 and divide B");
+        geo[i] = sm_geometry(q.A, q.B, q.H, q.W, q.spl);
     }
     SmGroup G;
     G.n = n;

@@ -146,7 +146,8 @@ __device__ __forceinline__ float huber_at(const float *Cs, int i, int j, int H, 
 
 // Slice-major back-projector (bwd_sm.hip): atten_sm (A, H, W, B), ratio_sm (A, W, B)
 // or null, one reference per line (em (A)); out/C/sens (B, H, W); mode a BwdMode.
-size_t sm_scratch_bytes(int64_t A, int64_t B, int64_t H, int64_t W);
+// spl: slices per lane (1, 2 or 4, dividing B), 0 for the default choice by B.
+size_t sm_scratch_bytes(int64_t A, int64_t B, int64_t H, int64_t W, int spl = 0);
 int launch_sm_backward(const float *atten_sm, const float *ratio_sm, const float *em, const float *theta,
                        float *out, int64_t A, int64_t B, int64_t H, int64_t W, int mode, const float *C,
                        const float *sens, float beta, float delta, void *scratch, size_t scratch_bytes,
@@ -167,7 +168,8 @@ struct SmFin;
 struct SmPack {
     const float *atten_sm, *ratio, *em, *theta;
     int64_t A, B, H, W;
-    void *scratch;  // sm_scratch_bytes(A, B, H, W)
+    int spl;        // slices per lane: 1, 2 or 4, dividing B
+    void *scratch;  // sm_scratch_bytes(A, B, H, W, spl)
 };
 struct SmFinSpec {  // one finalize variant of one pack
     float *out;

@@ -89,8 +89,11 @@ def test_packed_lines_match_oracle(beta, n_iter):
     """Problems sharing a theta tensor and (A, H, W) -- emission lines of one
     sample, with different slice counts and emission factors -- are packed
     along the slice axis in the slice-major layout: each line against the
-    oracle and against the reference layout; a lone problem beside them."""
-    groups = [(24, [3, 5, 1], 40, 52), (15, [2, 2], 33, 70), (10, [4], 20, 24)]
+    oracle and against the reference layout; a lone problem beside them, and
+    packs of 67 and 131 slices (two and four slices per lane, padded to 68 and
+    132)."""
+    groups = [(24, [3, 5, 1], 40, 52), (15, [2, 2], 33, 70), (10, [4], 20, 24), (6, [33, 34], 20, 24),
+              (5, [70, 61], 18, 22)]
     data, theta_of = [], []
     for g, (A, Bs, H, W) in enumerate(groups):
         first = len(data)
