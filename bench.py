@@ -59,6 +59,10 @@ def parse():
     ap.add_argument("--batch-samples", type=int, default=48,
                     help="samples of the batched uneven-instance workload (0: skip)")
     ap.add_argument("--batch-iters", type=int, default=20)
+    ap.add_argument("--batch-shapes", choices=("tuned", "default"), default="tuned",
+                    help="tile shapes of the batch: per size bucket from the autotuner, or the library defaults")
+    ap.add_argument("--no-batch-compare", action="store_true",
+                    help="time only the batch itself (no reference-layout, per-bucket or sequential runs)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--sequential-lines", action="store_true",
                     help="time the lines one after another on one stream (default: one stream per line, "
@@ -476,10 +480,13 @@ def cone_roofline(K, B, H, W, cone, reps=5):
 def batch_timings(args):
     """Batched uneven instances: seeded samples with lateral size in {64, 96, 128,
     160, 192}, 16-64 slices, 2-4 emission lines and 90-360 angles (multiples of
-    30); one problem per line.  Tile shapes per size bucket from the autotuner;
-    all problems iterate together, two iterations per replay of the captured
-    hipGraph.  The same problems run one after another through the per-problem
-    path (fl_mlem_ratio / fl_mlem_update launches) for comparison."""
+    30); one problem per line, the lines of a sample sharing its theta tensor.
+    Tile shapes per size bucket from the autotuner; all problems iterate
+    together, two iterations per replay of the captured hipGraph, in the
+    layout BatchMLEM picks (slice-major: each sample's lines packed along the
+    slice axis) and again in the reference layout; each size bucket alone
+    (its GB/s); and the same problems one after another through the
+    per-problem path (fl_mlem_ratio / fl_mlem_update launches)."""
     import tempfile
     import torch
     from fl_correction_amd.batch import Autotuner, BatchMLEM, Problem
@@ -487,6 +494,7 @@ def batch_timings(args):
     from fl_correction_amd.mlem import MLEMSolver
 
     dev = torch.device("cuda:0")
+    torch.cuda.empty_cache()  # the earlier legs' cached blocks: the batch workspaces are large
     rng = np.random.default_rng(20260003)
     gen = torch.Generator(device=dev)
     gen.manual_seed(20260003)
@@ -510,32 +518,79 @@ def batch_timings(args):
     atten_bytes = sum(4.0 * A * B * H * W for A, B, H, W in shapes_of)
     log(f"batch: {args.batch_samples} samples, {len(probs)} problems, {atten_bytes / 1e9:.1f} GB attenuation, "
         f"set up in {time.time() - t0:.1f}s")
-
-    t0 = time.time()
-    tuner = Autotuner(path=os.path.join(tempfile.mkdtemp(), "autotune.json"))
-    tile_shapes = [tuner.shapes(*s) for s in shapes_of]
-    log(f"batch: autotuned {len(tuner.table)} buckets in {time.time() - t0:.1f}s: {tuner.table}")
-
     n_it = args.batch_iters
-    run = BatchMLEM(probs, beta=0.0, shapes=tile_shapes)
-    run.run(2, graph=True)  # capture + first replay
-    for p in probs:
-        p.C.fill_(1.0)
-    torch.cuda.synchronize()
-    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-    e0.record(run.stream)
-    run.run(n_it, graph=True)
-    e1.record(run.stream)
-    torch.cuda.synchronize()
-    batch_ms = e0.elapsed_time(e1)
-    run.close()
+    tuner = Autotuner(path=os.path.join(tempfile.mkdtemp(), "autotune.json")) if args.batch_shapes == "tuned" \
+        else None
+
+    def timed(sel, layout, iters):
+        """ms per iteration of BatchMLEM over probs[sel] (graph replays), its
+        layout, and the set-up time in ms (slice-major copies, sensitivity)."""
+        sub = [probs[i] for i in sel]
+        for p in sub:
+            p.C.fill_(1.0)
+        torch.cuda.synchronize()
+        t_s = time.time()
+        run = BatchMLEM(sub, beta=0.0, shapes=tuner, layout=layout)
+        torch.cuda.synchronize()
+        setup_ms = (time.time() - t_s) * 1e3
+        run.run(2, graph=True)  # capture + first replay
+        for p in sub:
+            p.C.fill_(1.0)
+        torch.cuda.synchronize()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record(run.stream)
+        run.run(iters, graph=True)
+        e1.record(run.stream)
+        torch.cuda.synchronize()
+        lay = run.layout
+        run.close()
+        del run
+        torch.cuda.empty_cache()
+        return e0.elapsed_time(e1) / iters, lay, setup_ms
+
+    if tuner is not None:
+        t0 = time.time()
+        for s_ in shapes_of:  # tune every bucket in both layouts before timing
+            tuner.shapes(*s_, device=dev, layout="slice_major")
+            tuner.shapes(*s_, device=dev, layout="reference")
+        log(f"batch: autotuned {len(tuner.table)} (bucket, layout) pairs in {time.time() - t0:.1f}s: {tuner.table}")
+
+    every = list(range(len(probs)))
+    batch_ms, layout, setup_ms = timed(every, "auto", n_it)
     C_batch = [p.C.clone() for p in probs[:4]]
+    it_bytes = 2 * atten_bytes  # atten read by the forward and by the back-projector
+    out = {
+        "workload": "batched uneven MLEM: samples with lateral size in {64..192}, 16-64 slices, 2-4 lines, "
+                    "90-360 angles; one problem per line; beta=0",
+        "samples": args.batch_samples, "problems": len(shapes_of), "voxels": n_vox,
+        "attenuation_gb": atten_bytes / 1e9, "iterations": n_it,
+        "layout": layout,
+        "ms_per_iteration": batch_ms,
+        "atten_stream_gbs": it_bytes / (batch_ms * 1e-3) / 1e9,
+        "setup_ms": setup_ms,
+        "graph": "two iterations per replay",
+        "tile_shapes": {k.split("|", 1)[1]: v for k, v in tuner.table.items()} if tuner else "library defaults",
+    }
+    log(f"batch: {n_it} iterations {batch_ms:.1f} ms per iteration ({layout})")
+    if args.no_batch_compare:
+        return out
+    ref_ms, _, _ = timed(every, "reference", n_it)
+    buckets = {}
+    for L in sorted({s_[2] for s_ in shapes_of}):
+        sel = [i for i, s_ in enumerate(shapes_of) if s_[2] == L]
+        gb = sum(4.0 * A * B * H * W for A, B, H, W in (shapes_of[i] for i in sel)) / 1e9
+        ms, _, _ = timed(sel, layout, max(4, n_it // 4))
+        buckets[f"{L}x{L}"] = {"problems": len(sel), "attenuation_gb": gb, "ms_per_iteration": ms,
+                               "atten_stream_gbs": 2 * gb / (ms * 1e-3)}
+    log(f"batch: reference layout {ref_ms:.1f} ms; per bucket "
+        + ", ".join(f"{k} {v['atten_stream_gbs']:.0f} GB/s" for k, v in buckets.items()))
 
     # the same problems one after another (default tile shapes, one launch per pass per problem)
     solvers = [MLEMSolver(p.atten, p.em_cs[:, None].contiguous(), p.theta, p.I, layout="reference") for p in probs]
     Cs = [torch.ones_like(p.C)[None] for p in probs]
     Cn = [torch.empty_like(c) for c in Cs]
     torch.cuda.synchronize()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
     e0.record()
     for _ in range(n_it):
         for i, sv in enumerate(solvers):
@@ -548,21 +603,15 @@ def batch_timings(args):
     agree = max(float((a - b[0]).norm() / b[0].norm()) for a, b in zip(C_batch, Cs[:4]))
     del solvers, Cs, Cn, probs
     torch.cuda.empty_cache()
-    it_bytes = 2 * atten_bytes  # atten read by the forward and by the back-projector
-    log(f"batch: {n_it} iterations {batch_ms:.1f} ms (graph), sequential {seq_ms:.1f} ms")
-    return {
-        "workload": "batched uneven MLEM: samples with lateral size in {64..192}, 16-64 slices, 2-4 lines, "
-                    "90-360 angles; one problem per line; beta=0",
-        "samples": args.batch_samples, "problems": len(shapes_of), "voxels": n_vox,
-        "attenuation_gb": atten_bytes / 1e9, "iterations": n_it,
-        "ms_per_iteration": batch_ms / n_it,
+    log(f"batch: sequential {seq_ms / n_it:.1f} ms per iteration")
+    out.update({
+        "reference_layout_ms_per_iteration": ref_ms,
         "sequential_ms_per_iteration": seq_ms / n_it,
-        "speedup_vs_sequential": seq_ms / batch_ms,
-        "atten_stream_gbs": it_bytes / (batch_ms / n_it * 1e-3) / 1e9,
-        "graph": "two iterations per replay",
-        "tile_shapes": {k.split("|")[1]: v for k, v in tuner.table.items()},
+        "speedup_vs_sequential": seq_ms / n_it / batch_ms,
+        "per_bucket": buckets,
         "rel_l2_batch_vs_sequential_first4": agree,
-    }
+    })
+    return out
 
 
 def cpu_attenuation_sample(args):

@@ -10,10 +10,13 @@ of an iteration as one persistent launch over all problems and replay a
 captured hipGraph of two iterations.
 
 Tile shapes are chosen per problem by ``Autotuner``: for each size bucket
-(H and W rounded up to multiples of 32) it times full iterations of a
-synthetic batch of that bucket under every combination of forward angles per
-workgroup (8, 4) and back-projector tile height (31, 15), keeps the fastest,
-and caches the choice in a JSON file keyed by device name and bucket.
+(H and W rounded up to multiples of 32) and layout it times full iterations
+of a synthetic batch of that bucket -- with the reference layout under every
+combination of forward angles per workgroup (8, 4) and back-projector tile
+height (31, 15); with the slice-major layout, whose back-projector shape
+follows from each pack's slice count, under each forward angle count on
+three lines of one sample -- keeps the fastest, and caches the choice in a
+JSON file keyed by device name, layout and bucket.
 """
 from __future__ import annotations
 
@@ -96,20 +99,15 @@ class BatchMLEM:
         self.device = problems[0].atten.device
         if any(p.atten.device != self.device for p in problems):
             raise ValueError("all problems must be on one device")
-        if isinstance(shapes, Autotuner):
-            shapes = [shapes.shapes(*p.dims(), device=self.device) for p in problems]
-        self.shapes = list(shapes) if shapes is not None else [(0, 0)] * len(problems)
-        if len(self.shapes) != len(problems):
-            raise ValueError("one tile shape per problem")
+        if layout not in ("auto", "slice_major", "reference"):
+            raise ValueError(f"layout must be 'auto', 'slice_major' or 'reference', not {layout!r}")
         self.problems = list(problems)  # keeps every array alive while the plan uses it
-        self._arr = _structs(self.problems, self.shapes)
+        # the workspace sizes do not depend on the tile shapes
         lib = _lib.load()
-        arr = ctypes.cast(self._arr, ctypes.c_void_p)
+        arr = ctypes.cast(_structs(self.problems, [(0, 0)] * len(problems)), ctypes.c_void_p)
         nbytes = int(lib.fl_mlem_multi_workspace_bytes(arr, len(problems)))
         if nbytes == 0:
             raise ValueError("invalid batch")
-        if layout not in ("auto", "slice_major", "reference"):
-            raise ValueError(f"layout must be 'auto', 'slice_major' or 'reference', not {layout!r}")
         sm_bytes = int(lib.fl_mlem_multi_workspace_bytes_sm(arr, len(problems)))
         if layout == "auto":
             free, _ = torch.cuda.mem_get_info(self.device)
@@ -119,6 +117,12 @@ class BatchMLEM:
                 raise RuntimeError("the slice-major layout does not apply: a pack has H*W*sum(B) >= 2^29")
             nbytes = sm_bytes
         self.layout = layout
+        if isinstance(shapes, Autotuner):
+            shapes = [shapes.shapes(*p.dims(), device=self.device, layout=layout) for p in problems]
+        self.shapes = list(shapes) if shapes is not None else [(0, 0)] * len(problems)
+        if len(self.shapes) != len(problems):
+            raise ValueError("one tile shape per problem")
+        self._arr = _structs(self.problems, self.shapes)
         # 256-byte aligned workspace inside a slightly larger buffer
         self._ws_buf = torch.empty(nbytes + 256, dtype=torch.uint8, device=self.device)
         off = (-self._ws_buf.data_ptr()) % 256
@@ -168,9 +172,12 @@ class Autotuner:
     """Per size bucket tile shapes for the batched MLEM kernels.
 
     The bucket of a problem is (H, W) rounded up to multiples of 32.  A bucket
-    is tuned once per device: a synthetic batch of that bucket (enough slices
-    to fill the GPU several times over) runs ``reps`` timed iterations under
-    each (fwd_angles, bwd_rows) pair; the fastest is cached in ``path``.
+    is tuned once per device and layout: a synthetic batch of that bucket
+    (enough slices to fill the GPU several times over) runs ``reps`` timed
+    iterations under each candidate; the fastest is cached in ``path``.
+    Reference layout: every (fwd_angles, bwd_rows) pair.  Slice-major layout:
+    each fwd_angles (bwd_rows is unused there and recorded as 31), on three
+    emission lines sharing one theta, which the plan packs.
     """
 
     def __init__(self, path=None, reps=3, tune_angles=120):
@@ -190,15 +197,17 @@ class Autotuner:
         return (-(-int(H) // 32) * 32, -(-int(W) // 32) * 32)
 
     @staticmethod
-    def key(device_name, H, W):
+    def key(device_name, H, W, layout="reference"):
         bh, bw = Autotuner.bucket(H, W)
-        return f"{device_name}|{bh}x{bw}"
+        return f"{device_name}|{layout}|{bh}x{bw}"
 
-    def shapes(self, A, B, H, W, device=None):
+    def shapes(self, A, B, H, W, device=None, layout="reference"):
+        if layout not in ("slice_major", "reference"):
+            raise ValueError(f"layout must be 'slice_major' or 'reference', not {layout!r}")
         device = torch.device(device or "cuda")
-        k = self.key(torch.cuda.get_device_name(device), H, W)
+        k = self.key(torch.cuda.get_device_name(device), H, W, layout)
         if not self.valid(self.table.get(k)):  # missing, or from another library version
-            self.table[k] = list(self._tune(*self.bucket(H, W), device))
+            self.table[k] = list(self._tune(*self.bucket(H, W), device, layout))
             self._save()
         fa, br = self.table[k]
         return int(fa), int(br)
@@ -220,7 +229,7 @@ class Autotuner:
         except OSError:
             pass  # the choice is still used for this process
 
-    def _tune(self, H, W, device):
+    def _tune(self, H, W, device, layout="reference"):
         A = self.tune_angles
         # slices so that the back-projector has >= 4 tiles per resident workgroup
         props = torch.cuda.get_device_properties(device)
@@ -232,21 +241,22 @@ class Autotuner:
         theta = torch.arange(A, device=device, dtype=torch.float32) * (3.14159265 / A)
         I = torch.rand((A, B, W), generator=g, device=device) + 0.5
         C0 = torch.ones((B, H, W), device=device)
+        lines = 3 if layout == "slice_major" else 1
+        cands = [(fa, br) for fa in FWD_ANGLES for br in (BWD_ROWS if layout == "reference" else BWD_ROWS[:1])]
         best, best_ms = None, float("inf")
-        for fa in FWD_ANGLES:
-            for br in BWD_ROWS:
-                prob = Problem(atten, em, theta, I, C0.clone())
-                run = BatchMLEM([prob], shapes=[(fa, br)])
-                run.run(1, graph=False)
-                start = torch.cuda.Event(enable_timing=True)
-                stop = torch.cuda.Event(enable_timing=True)
-                start.record(run.stream)
-                run.run(2 * self.reps, graph=True)
-                stop.record(run.stream)
-                stop.synchronize()
-                ms = start.elapsed_time(stop) / (2 * self.reps)
-                run.close()
-                self.timings[(H, W, fa, br)] = ms
-                if ms < best_ms:
-                    best, best_ms = (fa, br), ms
+        for fa, br in cands:
+            probs = [Problem(atten, em, theta, I, C0.clone()) for _ in range(lines)]
+            run = BatchMLEM(probs, shapes=[(fa, br)] * lines, layout=layout)
+            run.run(1, graph=False)
+            start = torch.cuda.Event(enable_timing=True)
+            stop = torch.cuda.Event(enable_timing=True)
+            start.record(run.stream)
+            run.run(2 * self.reps, graph=True)
+            stop.record(run.stream)
+            stop.synchronize()
+            ms = start.elapsed_time(stop) / (2 * self.reps)
+            run.close()
+            self.timings[(layout, H, W, fa, br)] = ms
+            if ms < best_ms:
+                best, best_ms = (fa, br), ms
         return best

@@ -102,7 +102,8 @@ def test_autotuner_buckets_and_cache(tmp_path):
     from fl_correction_amd.batch import Autotuner
     assert Autotuner.bucket(64, 64) == (64, 64)
     assert Autotuner.bucket(65, 97) == (96, 128)
-    assert Autotuner.key("MI355X", 100, 160) == "MI355X|128x160"
+    assert Autotuner.key("MI355X", 100, 160) == "MI355X|reference|128x160"
+    assert Autotuner.key("MI355X", 100, 160, "slice_major") == "MI355X|slice_major|128x160"
     path = tmp_path / "t.json"
     path.write_text(json.dumps({"X|64x64": [4, 15]}))
     tu = Autotuner(path=str(path))

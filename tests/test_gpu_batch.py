@@ -181,10 +181,16 @@ def test_autotuner_picks_and_caches(tmp_path):
     tu = Autotuner(path=str(path), reps=1, tune_angles=24)
     fa, br = tu.shapes(24, 2, 40, 40)
     assert fa in FWD_ANGLES and br in BWD_ROWS
-    print("autotune 64x64 bucket:", tu.timings)
     assert len(tu.timings) == len(FWD_ANGLES) * len(BWD_ROWS)
+    # the slice-major layout tunes the forward angles only
+    fs, bs = tu.shapes(24, 2, 40, 40, layout="slice_major")
+    assert fs in FWD_ANGLES and bs == BWD_ROWS[0]
+    assert len(tu.timings) == len(FWD_ANGLES) * (len(BWD_ROWS) + 1)
+    print("autotune 64x64 bucket:", tu.timings)
     again = Autotuner(path=str(path))
     assert again.shapes(30, 1, 33, 64) == (fa, br)  # same bucket, from the cache
+    assert again.shapes(30, 1, 33, 64, layout="slice_major") == (fs, bs)
+    assert again.timings == {}
 
 
 @pytest.mark.parametrize("layout", LAYOUTS)
