@@ -70,18 +70,29 @@ __device__ __forceinline__ Box fwd_footprint(float c, float s, int h0, int h1, i
 }
 
 // One workgroup's tile t: detector strip t % ntw of angle group (t / ntw) %
-// n_groups of slice t / (ntw * n_groups); G angles per group.
-template <int NR, int G>
+// n_groups of slice t / (ntw * n_groups); G angles per group.  SLICE_FIRST
+// (the slice-major output) runs the slice fastest instead: workgroups that run
+// together then write neighbouring words of each (a, w) row of out, whose
+// cache lines fill in L2 before they are written back.
+template <int NR, int G, bool SLICE_FIRST = false>
 __device__ __forceinline__ void fwd_strip_tile(const FwdArgs &p, uint32_t t)
 {
     extern __shared__ float lds[];
     __shared__ float trig[2 * G];
     __shared__ float emv[G * kFRefChunk];
 
-    const int tw = t % p.ntw;
-    t /= p.ntw;
-    const int g = t % p.n_groups;
-    const int b = t / p.n_groups;
+    int tw, g, b;
+    if constexpr (SLICE_FIRST) {
+        b = (int)(t % (uint32_t)p.B);
+        t /= (uint32_t)p.B;
+        tw = (int)(t % (uint32_t)p.ntw);
+        g = (int)(t / (uint32_t)p.ntw);
+    } else {
+        tw = t % p.ntw;
+        t /= p.ntw;
+        g = t % p.n_groups;
+        b = t / p.n_groups;
+    }
 
     const int lane = threadIdx.x & 63;
     const int wave = uniform_int(threadIdx.x >> 6);
@@ -150,7 +161,8 @@ __device__ __forceinline__ void fwd_strip_tile(const FwdArgs &p, uint32_t t)
             while (gj < na) {
                 const int mrlo = min(rlo, lane_value(mine.rlo, gj)), mrhi = max(rhi, lane_value(mine.rhi, gj));
                 const int mclo = min(clo, lane_value(mine.clo, gj)), mchi = max(chi, lane_value(mine.chi, gj));
-                if ((mrhi - mrlo + 1) * ((mchi - mclo + 1) | 1) > kFLdsRef || mchi - mclo + 1 > kFStageCols) break;
+                if ((mrhi - mrlo + 1) * ((mchi - mclo + 1) | 1) > fwd_lds_ref(NR) || mchi - mclo + 1 > kFStageCols)
+                    break;
                 rlo = mrlo;
                 rhi = mrhi;
                 clo = mclo;
@@ -265,10 +277,10 @@ __device__ __forceinline__ void fwd_strip_tile(const FwdArgs &p, uint32_t t)
     }
 }
 
-template <int NR, int G>
+template <int NR, int G, bool SLICE_FIRST>
 __global__ __launch_bounds__(256, 2) void fwd_strip_kernel(FwdArgs p)
 {
-    fwd_strip_tile<NR, G>(p, xcd_remap(blockIdx.x, gridDim.x));
+    fwd_strip_tile<NR, G, SLICE_FIRST>(p, xcd_remap(blockIdx.x, gridDim.x));
 }
 
 // ---------------------------------------------------------------------------
@@ -1004,10 +1016,17 @@ __global__ __launch_bounds__(256, 2) void fwd_group_kernel(const FwdArgs *jobs, 
         const FwdArgs p = scalar_record(jobs + j);
         const uint32_t lt = t - scalar_read_u32(begin, j);
         if (t != w) __syncthreads();
-        if (p.group_angles == 4)
-            fwd_strip_tile<1, 4>(p, lt);
-        else
-            fwd_strip_tile<1, 8>(p, lt);
+        if (p.out_sm) {
+            if (p.group_angles == 4)
+                fwd_strip_tile<1, 4, true>(p, lt);
+            else
+                fwd_strip_tile<1, 8, true>(p, lt);
+        } else {
+            if (p.group_angles == 4)
+                fwd_strip_tile<1, 4>(p, lt);
+            else
+                fwd_strip_tile<1, 8>(p, lt);
+        }
     }
 }
 
@@ -1062,10 +1081,12 @@ int launch_forward(const float *atten, const float *C, const float *em, const fl
     static bool attr_set = false;
     const size_t shmem1 = fwd_lds_bytes(1), shmem2 = fwd_lds_bytes(2);
     if (!attr_set) {
-        FL_HIP_CHECK(hipFuncSetAttribute((const void *)fwd_strip_kernel<1, kFG>, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                         (int)shmem1));
-        FL_HIP_CHECK(hipFuncSetAttribute((const void *)fwd_strip_kernel<2, kFG>, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                         (int)shmem2));
+        const void *k1[2] = {(const void *)fwd_strip_kernel<1, kFG, false>, (const void *)fwd_strip_kernel<1, kFG, true>};
+        const void *k2[2] = {(const void *)fwd_strip_kernel<2, kFG, false>, (const void *)fwd_strip_kernel<2, kFG, true>};
+        for (int i = 0; i < 2; ++i) {
+            FL_HIP_CHECK(hipFuncSetAttribute(k1[i], hipFuncAttributeMaxDynamicSharedMemorySize, (int)shmem1));
+            FL_HIP_CHECK(hipFuncSetAttribute(k2[i], hipFuncAttributeMaxDynamicSharedMemorySize, (int)shmem2));
+        }
         attr_set = true;
     }
     FwdArgs p{};
@@ -1093,10 +1114,15 @@ int launch_forward(const float *atten, const float *C, const float *em, const fl
         const bool ratio = mode == kFwdRatio;
         p.mode = first ? (last && ratio ? kFwdRatio : kFwdWrite)
                        : (last && ratio ? kFwdAccumulateRatio : kFwdAccumulate);
-        if (p.nref == 2)
-            fwd_strip_kernel<2, kFG><<<dim3((uint32_t)nwg), dim3(256), shmem2, stream>>>(p);
+        const dim3 grid((uint32_t)nwg), block(256);
+        if (p.nref == 2 && out_sm)
+            fwd_strip_kernel<2, kFG, true><<<grid, block, shmem2, stream>>>(p);
+        else if (p.nref == 2)
+            fwd_strip_kernel<2, kFG, false><<<grid, block, shmem2, stream>>>(p);
+        else if (out_sm)
+            fwd_strip_kernel<1, kFG, true><<<grid, block, shmem1, stream>>>(p);
         else
-            fwd_strip_kernel<1, kFG><<<dim3((uint32_t)nwg), dim3(256), shmem1, stream>>>(p);
+            fwd_strip_kernel<1, kFG, false><<<grid, block, shmem1, stream>>>(p);
         FL_LAUNCH_CHECK();
     }
     return FL_OK;

@@ -18,14 +18,15 @@ constexpr int kFG = 8;                    // angles per workgroup (one accumulat
 // hypot(31, 63) < 70.3 rows (and cols); with the 0.5 margins and the bilinear
 // neighbour that is <= 74 rows and a pitch <= 75.
 constexpr int kFMaxFootprint = 74 * 75;
-// Footprint floats staged per reference (24 KB): one angle always fits, and the
-// union of a group of nearby angles usually does; the small budget keeps four
-// workgroups per CU for one reference.
-constexpr int kFLdsRef = 6144;
-static_assert(kFMaxFootprint <= kFLdsRef, "one angle's footprint must fit");
-static_assert(kFG * 4 * 64 <= kFLdsRef, "the final reduction reuses the staging area");
+// Footprint floats staged per reference: one angle always fits, and the union
+// of a group of nearby angles usually does.  One reference: 32 KB, still four
+// workgroups per CU (the registers allow four), and fewer unions split than
+// with 24 KB (5.42 vs 5.68 ms at 256^3 x 360).  Two references: 24 KB each.
+constexpr int fwd_lds_ref(int nref) { return nref == 1 ? 8192 : 6144; }
+static_assert(kFMaxFootprint <= fwd_lds_ref(2) && kFMaxFootprint <= fwd_lds_ref(1), "one angle's footprint must fit");
+static_assert(kFG * 4 * 64 <= fwd_lds_ref(2), "the final reduction reuses the staging area");
 constexpr int kFRefChunk = 2;             // references staged together
-constexpr size_t fwd_lds_bytes(int nref) { return sizeof(float) * (size_t)nref * kFLdsRef; }
+constexpr size_t fwd_lds_bytes(int nref) { return sizeof(float) * (size_t)nref * fwd_lds_ref(nref); }
 // Footprint columns the staging covers (two per lane); a union of several
 // angles' footprints may not be wider.  One angle's is at most 75.
 constexpr int kFStageCols = 2 * 64;
