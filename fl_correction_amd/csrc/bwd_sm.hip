@@ -13,29 +13,31 @@
 //
 // The detector points are partitioned by where the forward projector's sample
 // of each lands, and every point is taken by exactly one work item:
-//  * Interior blocks (sm_interior_kernel): points whose sample lies inside the
+//  * Interior blocks (sm_interior_body): points whose sample lies inside the
 //    clamp box [0, H-1.001] x [0, W-1.001] and whose base cell (floor of the
-//    position) is in one 4 x 4 block of base cells.  Lane 4c + q tests candidate
-//    q of cell c (the 2 x 2 lattice points of the bounding box of the cell's
-//    preimage, which hold every point of the cell); a quad permute then packs
-//    each cell's hits into its first lanes.  The block's 5 x 5 window of voxels
-//    (cells plus their lower/right neighbours) is 25 * SPL registers per lane,
-//    and every hit's four targets are registers named at compile time by its
-//    cell.  Hits are taken in passes (pass s = the s-th hit of every cell,
-//    usually two passes); a pass walks the 16 cells in a fixed unrolled order
-//    with no branches, loads issued a ring of cells ahead of their multiply-adds
-//    (a cell without a hit in the pass reads through an out-of-range offset,
-//    which returns 0).  The MLEM ratio rows the block needs at an angle are
-//    cached in LDS.  The 4 x 4 part of the window is the block's own output; its
-//    last row and column (the halo) go to a small buffer.
-//  * Edge strips (sm_edge_kernel): points whose sample the reference clamps onto
+//    position) is in one 4 x 4 block of base cells.  One wavefront per block:
+//    at each angle lane 4c + q tests candidate q of cell c (the 2 x 2 lattice
+//    points of the bounding box of the cell's preimage, which hold every point
+//    of the cell), and the wave's hit mask lists that angle's points.  The hits
+//    stream through a ring of 8 load slots across angles (the next angle with
+//    hits is tested ahead, so the ring does not drain between angles): each
+//    slot holds one point's attenuation and ratio loads and its weights, read
+//    from the hit's lane.  The block's 5 x 5 window of voxels (cells plus their
+//    lower/right neighbours) is 25 * SPL registers per lane, and a tree of
+//    wave-uniform branches on the hit's cell selects its four target registers,
+//    named at compile time.  The 4 x 4 part of the window is the block's own
+//    output; its last row and column (the halo) go to a small buffer.
+//  * Edge strips (sm_edge_body): points whose sample the reference clamps onto
 //    the image border (.cu:419-420), split by the border they clamp onto
 //    (row < 0: top, row > H-1.001: bottom, else col < 0: left, col > W-1.001:
 //    right), by an 8-wide strip of border base cells, by 8-voxel depth bins and
 //    by ranges of angles, so that no item holds a long chain of work.  A strip
 //    enumerates the lattice points of the bounding box of its region's preimage,
-//    64 per round, and read-add-writes its 2 x 9 (or 9 x 2) window of border
-//    voxels in LDS with the clamped position's weights.
+//    64 per round, its four wavefronts taking every fourth angle, and adds into
+//    its 2 x 9 (or 9 x 2) window of border voxels in registers with the clamped
+//    position's weights; the wavefronts' windows are summed in LDS in a fixed
+//    order.  Edge items and interior blocks share one launch (edge items first),
+//    so that the longer edge items overlap the blocks.
 // sm_finalize adds, per voxel and in a fixed order, the owning block's sum, the
 // halos of the blocks above and to the left, and every edge strip covering the
 // voxel, transposes back to the reference's (B, H, W) layout through LDS and
@@ -444,11 +446,6 @@ __device__ __forceinline__ void sm_edge_body(const SmArgs &p, uint32_t id)
     for (int q = wave; q < kSmEdgeVox; q += kSmEW) e[q * 64 + lane] = win[q * 64 + lane];
 }
 
-template <int SPL, bool RATIO>
-__global__ __launch_bounds__(64 * kSmEW) void sm_edge_kernel(SmArgs p)
-{
-    sm_edge_body<SPL, RATIO>(p, blockIdx.x);
-}
 
 // Sum of a strip's windows over its depth bins and angle ranges, in that fixed
 // order: one workgroup per (slice group, side, strip, window entry).
@@ -542,7 +539,7 @@ __device__ __forceinline__ void sm_interior_body(const SmArgs &p, uint32_t id)
     const int grp = (int)(id / (uint32_t)nint);
     const int k = (int)(id % (uint32_t)nint);
     const int bi = k / g.nbj, bj = k % g.nbj;
-    const int lane = threadIdx.x;
+    const int lane = threadIdx.x & 63;
     const int b0 = grp * S + lane * SPL;  // this lane's first slice (B is a multiple of SPL)
     const float xmax = g.H - 1.001f, ymax = g.W - 1.001f;
     const float cx = (g.W - 1) * 0.5f, cy = (g.H - 1) * 0.5f;
@@ -672,10 +669,31 @@ __device__ __forceinline__ void sm_interior_body(const SmArgs &p, uint32_t id)
     for (int yy = 0; yy < kSmTB; ++yy) hal[(kSmWB + yy) * 64 + lane] = acc[yy * kSmWB + kSmTB];
 }
 
+// Interior blocks and edge strips in one launch: workgroups [0, nq8) take four
+// interior blocks each, one per wavefront, in the XCD-contiguous order of the
+// interior's own launch (nq8: the quads rounded up to a multiple of the XCD
+// count), the rest the edge items in their dispatch order.  The short edge
+// items then fill the slots that the interior's last blocks leave idle,
+// instead of starting only after all of them.
 template <int SPL, bool RATIO>
-__global__ __launch_bounds__(64) void sm_interior_kernel(SmArgs p)
+__device__ __forceinline__ void sm_fused_body(const SmArgs &p, uint32_t wg, uint32_t nwg)
 {
-    sm_interior_body<SPL, RATIO>(p, xcd_remap(blockIdx.x, gridDim.x));
+    const SmGeom &g = p.g;
+    const uint32_t nint = (uint32_t)g.Gi * g.nbi * g.nbj;
+    const uint32_t nq8 = ((nint + kSmEW - 1) / kSmEW + kNumXcd - 1) / kNumXcd * kNumXcd;
+    if (wg < nq8) {
+        const uint32_t blk = xcd_remap(wg, nq8) * kSmEW + (threadIdx.x >> 6);
+        if (blk < nint) sm_interior_body<SPL, RATIO>(p, blk);
+        return;
+    }
+    const uint32_t item = wg - nq8;
+    if (item < (uint32_t)g.Gi * (uint32_t)sm_edge_items(g)) sm_edge_body<SPL, RATIO>(p, item);
+}
+
+template <int SPL, bool RATIO>
+__global__ __launch_bounds__(64 * kSmEW) void sm_fused_kernel(SmArgs p)
+{
+    sm_fused_body<SPL, RATIO>(p, blockIdx.x, gridDim.x);
 }
 
 // ---------------------------------------------------------------------------
@@ -779,21 +797,14 @@ __global__ __launch_bounds__(64) void sm_finalize_kernel(SmFin f)
 // the workgroups [begin[j], begin[j+1]) and runs them exactly as its own
 // launch would.  One launch per kernel and slices-per-lane class.
 // ---------------------------------------------------------------------------
+// Job j owns workgroups [begin[j], begin[j+1]).
 template <int SPL>
-__global__ __launch_bounds__(64) void sm_interior_group_kernel(const SmArgs *jobs, const uint32_t *begin, int n)
-{
-    const uint32_t t = xcd_remap(blockIdx.x, gridDim.x);
-    const int j = find_job(begin, n, t);
-    const SmArgs p = scalar_record(jobs + j);
-    sm_interior_body<SPL, true>(p, t - scalar_read_u32(begin, j));
-}
-
-template <int SPL>
-__global__ __launch_bounds__(64 * kSmEW) void sm_edge_group_kernel(const SmArgs *jobs, const uint32_t *begin, int n)
+__global__ __launch_bounds__(64 * kSmEW) void sm_fused_group_kernel(const SmArgs *jobs, const uint32_t *begin, int n)
 {
     const int j = find_job(begin, n, blockIdx.x);
     const SmArgs p = scalar_record(jobs + j);
-    sm_edge_body<SPL, true>(p, blockIdx.x - scalar_read_u32(begin, j));
+    const uint32_t b0 = scalar_read_u32(begin, j);
+    sm_fused_body<SPL, true>(p, blockIdx.x - b0, scalar_read_u32(begin, j + 1) - b0);
 }
 
 template <int SPL>
@@ -963,19 +974,24 @@ static SmFin sm_fin(const SmArgs &p, float *out, int mode, const float *C, const
 // Workgroups of each launch of one problem.
 static int64_t sm_interior_wgs(const SmGeom &g) { return (int64_t)g.Gi * g.nbi * g.nbj; }
 static int64_t sm_edge_wgs(const SmGeom &g) { return (int64_t)g.Gi * sm_edge_items(g); }
+// Fused interior + edge launch: quads of interior blocks, then the edge items,
+// each padded to a multiple of the XCD count (in a grouped launch every job
+// then starts on XCD 0, as its own launch would).
+static int64_t sm_fused_wgs(const SmGeom &g)
+{
+    const int64_t x = kNumXcd;
+    return ((sm_interior_wgs(g) + kSmEW - 1) / kSmEW + x - 1) / x * x + (sm_edge_wgs(g) + x - 1) / x * x;
+}
 static int64_t sm_reduce_wgs(const SmGeom &g) { return (int64_t)g.Gi * 4 * std::max(g.nsi, g.nsj) * kSmEdgeVox; }
 static int64_t sm_finalize_wgs(const SmGeom &g) { return (int64_t)((g.W + 63) / 64) * g.H * ((g.B + 63) / 64); }
 
 template <int SPL>
-static void sm_launch(const SmArgs &p, uint32_t n_int, uint32_t n_edge, uint32_t n_red, hipStream_t stream)
+static void sm_launch(const SmArgs &p, uint32_t n_fused, uint32_t n_red, hipStream_t stream)
 {
-    if (p.ratio) {
-        sm_interior_kernel<SPL, true><<<dim3(n_int), dim3(64), 0, stream>>>(p);
-        sm_edge_kernel<SPL, true><<<dim3(n_edge), dim3(64 * kSmEW), 0, stream>>>(p);
-    } else {
-        sm_interior_kernel<SPL, false><<<dim3(n_int), dim3(64), 0, stream>>>(p);
-        sm_edge_kernel<SPL, false><<<dim3(n_edge), dim3(64 * kSmEW), 0, stream>>>(p);
-    }
+    if (p.ratio)
+        sm_fused_kernel<SPL, true><<<dim3(n_fused), dim3(64 * kSmEW), 0, stream>>>(p);
+    else
+        sm_fused_kernel<SPL, false><<<dim3(n_fused), dim3(64 * kSmEW), 0, stream>>>(p);
     sm_edge_reduce_kernel<SPL><<<dim3(n_red), dim3(64), 0, stream>>>(p);
 }
 
@@ -988,17 +1004,15 @@ int launch_sm_backward(const float *atten_sm, const float *ratio_sm, const float
     FL_REQUIRE(scratch_bytes >= sm_scratch_bytes(A, B, H, W) && scratch, "slice-major back-projector: scratch too small");
     FL_REQUIRE(H * W * B < ((int64_t)1 << 29), "slice-major back-projector: H*W*B must be below 2^29");
     const SmArgs p = sm_args(g, atten_sm, ratio_sm, em, theta, scratch);
-    const int64_t n_int = sm_interior_wgs(g);
-    const int64_t n_e = sm_edge_wgs(g);
+    const int64_t n_fused = sm_fused_wgs(g);
     const int64_t n_red = sm_reduce_wgs(g);
-    FL_REQUIRE(n_int < (int64_t)UINT32_MAX && n_e < (int64_t)UINT32_MAX && n_red < (int64_t)UINT32_MAX,
-               "slice-major back-projector: grid too large");
+    FL_REQUIRE(n_fused < ((int64_t)1 << 24) && n_red < ((int64_t)1 << 26), "slice-major back-projector: grid too large");
     if (g.spl == 4)
-        sm_launch<4>(p, (uint32_t)n_int, (uint32_t)n_e, (uint32_t)n_red, stream);
+        sm_launch<4>(p, (uint32_t)n_fused, (uint32_t)n_red, stream);
     else if (g.spl == 2)
-        sm_launch<2>(p, (uint32_t)n_int, (uint32_t)n_e, (uint32_t)n_red, stream);
+        sm_launch<2>(p, (uint32_t)n_fused, (uint32_t)n_red, stream);
     else
-        sm_launch<1>(p, (uint32_t)n_int, (uint32_t)n_e, (uint32_t)n_red, stream);
+        sm_launch<1>(p, (uint32_t)n_fused, (uint32_t)n_red, stream);
     FL_LAUNCH_CHECK();
     const SmFin f = sm_fin(p, out, mode, C, sens, beta, delta);
     sm_finalize_kernel<<<dim3((uint32_t)((W + 63) / 64), (uint32_t)H, (uint32_t)((B + 63) / 64)), dim3(64), 0,
@@ -1012,13 +1026,14 @@ int launch_sm_backward(const float *atten_sm, const float *ratio_sm, const float
 // ---------------------------------------------------------------------------
 static size_t tb_align(size_t b) { return (b + 255) / 256 * 256; }
 
-// Table image: SmArgs[n] (grouped by slices-per-lane class), three begin arrays
-// per class, then n_var SmFin[n] and the finalize begin array.
+// Table image: SmArgs[n] (grouped by slices-per-lane class), two begin arrays
+// per class (fused edge + interior, reduce), then n_var SmFin[n] and the
+// finalize begin array.
 size_t sm_group_table_bytes(int n)
 {
     if (n < 1) return 0;
     size_t b = tb_align(n * sizeof(SmArgs));
-    b += 3 * 3 * tb_align((size_t)(n + 1) * sizeof(uint32_t));
+    b += 3 * 2 * tb_align((size_t)(n + 1) * sizeof(uint32_t));
     b += kSmGroupVariants * tb_align(n * sizeof(SmFin));
     b += tb_align((size_t)(n + 1) * sizeof(uint32_t));
     return b;
@@ -1056,33 +1071,27 @@ int sm_group_build(const SmPack *packs, int n, const SmFinSpec *fin, int n_var, 
         cl.spl = spls[c];
         cl.n = (int)idx.size();
         cl.jobs = (size_t)k * sizeof(SmArgs);
-        cl.b_int = o;
-        cl.b_edge = o + begin_bytes;
-        cl.b_red = o + 2 * begin_bytes;
-        uint32_t *bi = (uint32_t *)(host + cl.b_int), *be = (uint32_t *)(host + cl.b_edge),
-                 *br = (uint32_t *)(host + cl.b_red);
-        int64_t ti = 0, te = 0, tr = 0;
+        cl.b_fused = o;
+        cl.b_red = o + begin_bytes;
+        uint32_t *bf = (uint32_t *)(host + cl.b_fused), *br = (uint32_t *)(host + cl.b_red);
+        int64_t tf = 0, tr = 0;
         for (size_t m = 0; m < idx.size(); ++m) {
             const SmPack &q = packs[idx[m]];
             jobs[k + m] = sm_args(geo[idx[m]], q.atten_sm, q.ratio, q.em, q.theta, q.scratch);
-            bi[m] = (uint32_t)ti;
-            be[m] = (uint32_t)te;
+            bf[m] = (uint32_t)tf;
             br[m] = (uint32_t)tr;
-            ti += sm_interior_wgs(geo[idx[m]]);
-            te += sm_edge_wgs(geo[idx[m]]);
+            tf += sm_fused_wgs(geo[idx[m]]);
             tr += sm_reduce_wgs(geo[idx[m]]);
         }
         // workgroups x threads must stay below 2^32
-        FL_REQUIRE(ti < ((int64_t)1 << 26) && te < ((int64_t)1 << 24) && tr < ((int64_t)1 << 26),
+        FL_REQUIRE(tf < ((int64_t)1 << 24) && tr < ((int64_t)1 << 26),
                    "slice-major group: too many workgroups for one launch");
-        bi[idx.size()] = (uint32_t)ti;
-        be[idx.size()] = (uint32_t)te;
+        bf[idx.size()] = (uint32_t)tf;
         br[idx.size()] = (uint32_t)tr;
-        cl.t_int = (uint32_t)ti;
-        cl.t_edge = (uint32_t)te;
+        cl.t_fused = (uint32_t)tf;
         cl.t_red = (uint32_t)tr;
         k += (int)idx.size();
-        o += 3 * begin_bytes;
+        o += 2 * begin_bytes;
     }
     // finalize variants in pack order
     for (int v = 0; v < n_var; ++v) {
@@ -1116,10 +1125,8 @@ template <int SPL>
 static void sm_group_class(const SmGroup &G, const SmGroup::Class &c, hipStream_t s)
 {
     const SmArgs *jobs = (const SmArgs *)(G.dev + c.jobs);
-    const uint32_t *bi = (const uint32_t *)(G.dev + c.b_int), *be = (const uint32_t *)(G.dev + c.b_edge),
-                   *br = (const uint32_t *)(G.dev + c.b_red);
-    if (c.t_int) sm_interior_group_kernel<SPL><<<dim3(c.t_int), dim3(64), 0, s>>>(jobs, bi, c.n);
-    if (c.t_edge) sm_edge_group_kernel<SPL><<<dim3(c.t_edge), dim3(64 * kSmEW), 0, s>>>(jobs, be, c.n);
+    const uint32_t *bf = (const uint32_t *)(G.dev + c.b_fused), *br = (const uint32_t *)(G.dev + c.b_red);
+    if (c.t_fused) sm_fused_group_kernel<SPL><<<dim3(c.t_fused), dim3(64 * kSmEW), 0, s>>>(jobs, bf, c.n);
     if (c.t_red) sm_edge_reduce_group_kernel<SPL><<<dim3(c.t_red), dim3(64), 0, s>>>(jobs, br, c.n);
 }
 

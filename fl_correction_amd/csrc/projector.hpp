@@ -182,8 +182,8 @@ constexpr int kSmGroupVariants = 3;
 struct SmGroup {
     struct Class {
         int spl = 0, n = 0;
-        size_t jobs = 0, b_int = 0, b_edge = 0, b_red = 0;  // byte offsets in the table image
-        uint32_t t_int = 0, t_edge = 0, t_red = 0;           // workgroups per launch
+        size_t jobs = 0, b_fused = 0, b_red = 0;  // byte offsets in the table image
+        uint32_t t_fused = 0, t_red = 0;           // workgroups per launch
     } cls[3];
     int n = 0, n_var = 0;
     size_t fin[kSmGroupVariants] = {}, b_fin = 0;
