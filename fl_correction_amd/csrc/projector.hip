@@ -71,9 +71,12 @@ __device__ __forceinline__ Box fwd_footprint(float c, float s, int h0, int h1, i
 
 // One workgroup's tile t: detector strip t % ntw of angle group (t / ntw) %
 // n_groups of slice t / (ntw * n_groups); G angles per group.  SLICE_FIRST
-// (the slice-major output) runs the slice fastest instead: workgroups that run
-// together then write neighbouring words of each (a, w) row of out, whose
-// cache lines fill in L2 before they are written back.
+// (the slice-major output) takes the slices in runs of kFSliceRun instead, the
+// slice fastest within a run: workgroups that run together then write whole
+// 32-byte sectors of each (a, w) row of out, and a run's few slices of C stay
+// in L2 while its strips and angle groups go by.  Slices past the last whole
+// run follow in the plain order.
+constexpr int kFSliceRun = 8;
 template <int NR, int G, bool SLICE_FIRST = false>
 __device__ __forceinline__ void fwd_strip_tile(const FwdArgs &p, uint32_t t)
 {
@@ -83,10 +86,22 @@ __device__ __forceinline__ void fwd_strip_tile(const FwdArgs &p, uint32_t t)
 
     int tw, g, b;
     if constexpr (SLICE_FIRST) {
-        b = (int)(t % (uint32_t)p.B);
-        t /= (uint32_t)p.B;
-        tw = (int)(t % (uint32_t)p.ntw);
-        g = (int)(t / (uint32_t)p.ntw);
+        const uint32_t per_slice = (uint32_t)p.ntw * (uint32_t)p.n_groups;
+        const uint32_t full = (uint32_t)p.B / kFSliceRun * kFSliceRun;
+        if (t < per_slice * full) {
+            const uint32_t lo = t % kFSliceRun;
+            t /= kFSliceRun;
+            tw = (int)(t % (uint32_t)p.ntw);
+            t /= (uint32_t)p.ntw;
+            g = (int)(t % (uint32_t)p.n_groups);
+            b = (int)((t / (uint32_t)p.n_groups) * kFSliceRun + lo);
+        } else {
+            t -= per_slice * full;
+            tw = (int)(t % (uint32_t)p.ntw);
+            t /= (uint32_t)p.ntw;
+            g = (int)(t % (uint32_t)p.n_groups);
+            b = (int)(full + t / (uint32_t)p.n_groups);
+        }
     } else {
         tw = t % p.ntw;
         t /= p.ntw;
