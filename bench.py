@@ -237,10 +237,10 @@ def main():
         upd_name = ("slice-major back-projection + MLEM update: sm_fused_kernel<4,true> (interior blocks and "
                     "edge strips), sm_edge_reduce_kernel<4>, sm_finalize_kernel (one library call)")
         upd_key = "slice-major back-projection + update"
-        fwd_name = "fwd_strip_kernel<1,8,true> (forward projection + ratio, written slice-major)"
+        fwd_name = "fwd_strip_kernel<1,8> (forward projection + ratio, written slice-major)"
     else:
         upd_name, upd_key = "bwd_tile_kernel<1,31> (back-projection + MLEM update)", "bwd_tile_kernel<1,31>"
-        fwd_name = "fwd_strip_kernel<1,8,false> (forward projection + ratio)"
+        fwd_name = "fwd_strip_kernel<1,8> (forward projection + ratio)"
     fwd_key = "fwd_strip_kernel<1,8>"
     if upd_avg >= fwd_avg:
         dom, dom_ms, dom_bytes, dom_key = upd_name, upd_avg, upd_bytes, upd_key

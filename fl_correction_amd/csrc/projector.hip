@@ -69,15 +69,16 @@ __device__ __forceinline__ Box fwd_footprint(float c, float s, int h0, int h1, i
     return b;
 }
 
-// One workgroup's tile t: detector strip t % ntw of angle group (t / ntw) %
-// n_groups of slice t / (ntw * n_groups); G angles per group.  SLICE_FIRST
-// (the slice-major output) takes the slices in runs of kFSliceRun instead, the
-// slice fastest within a run: workgroups that run together then write whole
-// 32-byte sectors of each (a, w) row of out, and a run's few slices of C stay
-// in L2 while its strips and angle groups go by.  Slices past the last whole
-// run follow in the plain order.
+// One workgroup's tile t: a strip of kFW detector columns, an angle group of G
+// angles and a slice.  Slices go in runs of kFSliceRun, the slice fastest
+// within a run, then the strip, then the angle group; slices past the last
+// whole run follow with the strip fastest.  Workgroups that run together then
+// share their strip's angles and footprints (their samples' positions are the
+// same), a run's few slices of C stay in L2 while its strips and angle groups
+// go by, and the slice-major output (A, W, B) is written in whole 32-byte
+// sectors.  At 256^3 x 360: 5.17 vs 5.66 ms with the plain strip-fastest order.
 constexpr int kFSliceRun = 8;
-template <int NR, int G, bool SLICE_FIRST = false>
+template <int NR, int G>
 __device__ __forceinline__ void fwd_strip_tile(const FwdArgs &p, uint32_t t)
 {
     extern __shared__ float lds[];
@@ -85,7 +86,7 @@ __device__ __forceinline__ void fwd_strip_tile(const FwdArgs &p, uint32_t t)
     __shared__ float emv[G * kFRefChunk];
 
     int tw, g, b;
-    if constexpr (SLICE_FIRST) {
+    {
         const uint32_t per_slice = (uint32_t)p.ntw * (uint32_t)p.n_groups;
         const uint32_t full = (uint32_t)p.B / kFSliceRun * kFSliceRun;
         if (t < per_slice * full) {
@@ -102,11 +103,6 @@ __device__ __forceinline__ void fwd_strip_tile(const FwdArgs &p, uint32_t t)
             g = (int)(t % (uint32_t)p.n_groups);
             b = (int)(full + t / (uint32_t)p.n_groups);
         }
-    } else {
-        tw = t % p.ntw;
-        t /= p.ntw;
-        g = t % p.n_groups;
-        b = t / p.n_groups;
     }
 
     const int lane = threadIdx.x & 63;
@@ -292,10 +288,10 @@ __device__ __forceinline__ void fwd_strip_tile(const FwdArgs &p, uint32_t t)
     }
 }
 
-template <int NR, int G, bool SLICE_FIRST>
+template <int NR, int G>
 __global__ __launch_bounds__(256, 2) void fwd_strip_kernel(FwdArgs p)
 {
-    fwd_strip_tile<NR, G, SLICE_FIRST>(p, xcd_remap(blockIdx.x, gridDim.x));
+    fwd_strip_tile<NR, G>(p, xcd_remap(blockIdx.x, gridDim.x));
 }
 
 // ---------------------------------------------------------------------------
@@ -1031,17 +1027,10 @@ __global__ __launch_bounds__(256, 2) void fwd_group_kernel(const FwdArgs *jobs, 
         const FwdArgs p = scalar_record(jobs + j);
         const uint32_t lt = t - scalar_read_u32(begin, j);
         if (t != w) __syncthreads();
-        if (p.out_sm) {
-            if (p.group_angles == 4)
-                fwd_strip_tile<1, 4, true>(p, lt);
-            else
-                fwd_strip_tile<1, 8, true>(p, lt);
-        } else {
-            if (p.group_angles == 4)
-                fwd_strip_tile<1, 4>(p, lt);
-            else
-                fwd_strip_tile<1, 8>(p, lt);
-        }
+        if (p.group_angles == 4)
+            fwd_strip_tile<1, 4>(p, lt);
+        else
+            fwd_strip_tile<1, 8>(p, lt);
     }
 }
 
@@ -1096,12 +1085,10 @@ int launch_forward(const float *atten, const float *C, const float *em, const fl
     static bool attr_set = false;
     const size_t shmem1 = fwd_lds_bytes(1), shmem2 = fwd_lds_bytes(2);
     if (!attr_set) {
-        const void *k1[2] = {(const void *)fwd_strip_kernel<1, kFG, false>, (const void *)fwd_strip_kernel<1, kFG, true>};
-        const void *k2[2] = {(const void *)fwd_strip_kernel<2, kFG, false>, (const void *)fwd_strip_kernel<2, kFG, true>};
-        for (int i = 0; i < 2; ++i) {
-            FL_HIP_CHECK(hipFuncSetAttribute(k1[i], hipFuncAttributeMaxDynamicSharedMemorySize, (int)shmem1));
-            FL_HIP_CHECK(hipFuncSetAttribute(k2[i], hipFuncAttributeMaxDynamicSharedMemorySize, (int)shmem2));
-        }
+        FL_HIP_CHECK(hipFuncSetAttribute((const void *)fwd_strip_kernel<1, kFG>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                         (int)shmem1));
+        FL_HIP_CHECK(hipFuncSetAttribute((const void *)fwd_strip_kernel<2, kFG>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                         (int)shmem2));
         attr_set = true;
     }
     FwdArgs p{};
@@ -1130,14 +1117,10 @@ int launch_forward(const float *atten, const float *C, const float *em, const fl
         p.mode = first ? (last && ratio ? kFwdRatio : kFwdWrite)
                        : (last && ratio ? kFwdAccumulateRatio : kFwdAccumulate);
         const dim3 grid((uint32_t)nwg), block(256);
-        if (p.nref == 2 && out_sm)
-            fwd_strip_kernel<2, kFG, true><<<grid, block, shmem2, stream>>>(p);
-        else if (p.nref == 2)
-            fwd_strip_kernel<2, kFG, false><<<grid, block, shmem2, stream>>>(p);
-        else if (out_sm)
-            fwd_strip_kernel<1, kFG, true><<<grid, block, shmem1, stream>>>(p);
+        if (p.nref == 2)
+            fwd_strip_kernel<2, kFG><<<grid, block, shmem2, stream>>>(p);
         else
-            fwd_strip_kernel<1, kFG, false><<<grid, block, shmem1, stream>>>(p);
+            fwd_strip_kernel<1, kFG><<<grid, block, shmem1, stream>>>(p);
         FL_LAUNCH_CHECK();
     }
     return FL_OK;
