@@ -41,18 +41,28 @@ def data_residual(Pf, I):
     return float(torch.linalg.vector_norm(Pf - I) / torch.linalg.vector_norm(I))
 
 
-def mlem_start(model: XRFModel, I, C_init, n_iter=10, beta=0.0, delta=0.01):
+def mlem_start(model: XRFModel, I, C_init, n_iter=10, beta=0.0, delta=0.01, per_line=None):
     """n_iter MLEM iterations per emission line, on the attenuation of C_init.
 
-    I (K, A, B, W), C_init (K, B, H, W); returns a new (K, B, H, W) tensor."""
+    I (K, A, B, W), C_init (K, B, H, W); returns a new (K, B, H, W) tensor.
+    per_line: compute and keep one line's attenuation (A, B, H, W) at a time
+    (each recompute runs the cone sweep for all lines) instead of all K lines'
+    at once; None chooses it when the K lines' maps would take more than a third
+    of the free device memory (5 lines at 720 x 128 x 512^2 take 483 GB)."""
     C_init = C_init.to(model.device, torch.float32).contiguous()
     I = I.to(model.device, torch.float32).contiguous()
-    atten = model.attenuation(C_init)  # (K, A, B, H, W)
+    K, B, H, W = C_init.shape
+    if per_line is None:
+        free, _ = torch.cuda.mem_get_info(model.device)
+        per_line = 4.0 * K * model.A * B * H * W > free / 3
     out = torch.empty_like(C_init)
+    atten = None if per_line else model.attenuation(C_init)  # (K, A, B, H, W)
     for e in range(model.K):
+        a_e = model.attenuation(C_init, lines=[e])[0] if per_line else atten[e]
         em = model.em_T[e][:, None].contiguous()  # (A, 1): one reference per line
-        solver = MLEMSolver(atten[e], em, model.theta, I[e].contiguous(), beta=beta, delta=delta)
+        solver = MLEMSolver(a_e, em, model.theta, I[e].contiguous(), beta=beta, delta=delta)
         out[e] = solver.run(C_init[e:e + 1], n_iter)[0]
+        del solver, a_e
     del atten
     return out
 

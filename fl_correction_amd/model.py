@@ -160,8 +160,11 @@ class XRFModel:
             cur.wait_event(done)
 
     # ------------------------------------------------------------------
-    def attenuation(self, C, out=None, streams=2):
-        """Total attenuation (K, A, B, H, W) for concentrations C (K, B, H, W).
+    def attenuation(self, C, out=None, streams=2, lines=None):
+        """Total attenuation (K, A, B, H, W) for concentrations C (K, B, H, W), or
+        (len(lines), A, B, H, W) for the emission lines listed in `lines` (every
+        line's exponent is still computed: the lines share the cone sweep; only
+        the listed lines' maps are written).
 
         The angles are independent: they are dealt round-robin to `streams`
         streams (the caller's current stream and default-priority side streams),
@@ -170,8 +173,12 @@ class XRFModel:
         caller's stream waits for the side streams before this returns."""
         C = C.to(self.device, torch.float32).contiguous()
         K, B, H, W = C.shape
+        sel = None if lines is None else [int(e) for e in lines]
+        if sel is not None and (not sel or any(e < 0 or e >= K for e in sel)):
+            raise ValueError(f"lines must be indices of the {K} emission lines")
         if out is None:
-            out = torch.empty((K, self.A, B, H, W), dtype=torch.float32, device=self.device)
+            out = torch.empty((K if sel is None else len(sel), self.A, B, H, W), dtype=torch.float32,
+                              device=self.device)
         n_st = max(1, min(int(streams), self.A))
         with torch.cuda.device(self.device):
             cur = torch.cuda.current_stream(self.device)
@@ -192,8 +199,13 @@ class XRFModel:
                     with torch.cuda.stream(sts[i]):
                         F, Ax, mu_rot, T_rot = bufs[i]
                         self._angle_maps(C, a, F, Ax, mu_rot, T_rot)
-                        _lib.call("fl_xrf_combine", _lib.ptr(T_rot), _lib.ptr(Ax), _lib.ptr(out[0, a]), K, B, H, W,
-                                  B * H * W, out.stride(0), self.side, self._st())
+                        if sel is None:
+                            _lib.call("fl_xrf_combine", _lib.ptr(T_rot), _lib.ptr(Ax), _lib.ptr(out[0, a]), K, B, H,
+                                      W, B * H * W, out.stride(0), self.side, self._st())
+                        else:
+                            for j, e in enumerate(sel):
+                                _lib.call("fl_xrf_combine", _lib.ptr(T_rot[e]), _lib.ptr(Ax[e]), _lib.ptr(out[j, a]),
+                                          1, B, H, W, B * H * W, out.stride(0), self.side, self._st())
             finally:
                 # also on an error: the scratch returns to the caller's stream's pool only
                 # after the side streams are done with it
