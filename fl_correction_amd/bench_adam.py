@@ -11,7 +11,8 @@ a uniform start on the start's attenuation (one line's maps at a time when all
 lines' would not fit), then joint.joint_reconstruct runs --steps Adam steps,
 each one XRFModel.loss_and_grad over all angles.  Prints a line per phase and
 step, then one JSON line: the seconds of each phase, seconds per Adam step and
-the data residual ||Pf - I|| / ||I|| after the start and after each step.
+the data residual ||Pf - I|| / ||I|| after the start and after each step (and,
+with --plain-mlem N, after N plain MLEM iterations from the uniform start).
 """
 from __future__ import annotations
 
@@ -25,7 +26,7 @@ import torch
 from . import phantom
 from .atten_util import make_param
 from .detector_mask import Cone
-from .joint import joint_reconstruct, mlem_start
+from .joint import data_residual, joint_reconstruct, mlem_start
 from .model import XRFModel
 
 
@@ -40,6 +41,9 @@ def main(argv=None):
     ap.add_argument("--half-angle", type=float, default=15.0, help="cone half-angle in degrees")
     ap.add_argument("--mlem-iters", type=int, default=10)
     ap.add_argument("--steps", type=int, default=5, help="Adam steps timed")
+    ap.add_argument("--plain-mlem", type=int, default=0,
+                    help="also run this many plain MLEM iterations from the uniform start (on its attenuation) "
+                         "and report their data residual beside Adam's")
     ap.add_argument("--seed", type=int, default=20260005)
     args = ap.parse_args(argv)
 
@@ -90,6 +94,14 @@ def main(argv=None):
     C, hist = joint_reconstruct(model, I, C_mlem, n_steps=args.steps, callback=on_step)
     t_adam = sync_time() - t0
     per_step = float(np.mean(step_s)) if step_s else None
+    plain = {}
+    if args.plain_mlem > 0:
+        t0 = sync_time()
+        C_plain = mlem_start(model, I, C0, n_iter=args.plain_mlem)
+        t_plain = sync_time() - t0
+        res_plain = data_residual(model.forward(C_plain), I)
+        log(f"{args.plain_mlem} plain MLEM iterations: {t_plain:.1f} s, data residual {res_plain:.4f}")
+        plain = {"plain_mlem_iters": args.plain_mlem, "plain_mlem_s": t_plain, "plain_mlem_data_residual": res_plain}
     print(json.dumps({
         "volume": [B, N, N], "elements": K, "angles": A, "angle_span_deg": 360 if args.full_turn else 180,
         "cone": f"{args.cone}x{args.cone} directions, {args.half_angle:g} deg half-angle, slopes in 1/8",
@@ -98,6 +110,7 @@ def main(argv=None):
         "adam_100_steps_s_extrapolated": None if per_step is None else 100 * per_step,
         "data_residual_after_start": hist[0][2], "data_residual_per_step": [h[2] for h in hist[1:]],
         "loss_per_step": [h[1] for h in hist],
+        **plain,
         "peak_memory_gb": torch.cuda.max_memory_allocated() / 1e9,
         "finite": bool(torch.isfinite(C).all()),
     }))

@@ -177,3 +177,79 @@ def test_attenuation_of_selected_lines_and_per_line_mlem_start():
     a = mlem_start(model, I, C0, n_iter=3, per_line=False)
     b = mlem_start(model, I, C0, n_iter=3, per_line=True)
     assert torch.equal(a, b)
+
+
+def test_joint_reconstruction_against_fp64_oracle_and_plain_mlem():
+    """The joint-reconstruction judgement at a small size: a 10-iteration MLEM
+    start, then 100 Adam steps with C >= 0, beside 100 plain MLEM iterations on
+    the start's attenuation.  Both GPU results against the same pipeline in FP64
+    on the CPU (the torch FP64 model with the cone's splat mask, autograd and
+    torch's Adam; the oracle's FP64 MLEM on the FP64 attenuation of the start),
+    and the data residuals of all three."""
+    import torch_model as tm
+    from helpers import oracle, rel_l2
+    from fl_correction_amd import phantom
+    from fl_correction_amd.atten_util import make_param
+    from fl_correction_amd.detector_mask import Cone
+    from fl_correction_amd.joint import data_residual, joint_reconstruct, mlem_start
+    from fl_correction_amd.model import XRFModel
+    K, B, N, A = 2, 3, 16, 8
+    rng = np.random.default_rng(20260011)
+    C_true = phantom.ellipsoid_phantom((B, N, N), K, n_ell=(2, 4), semi_axes=(2, 6), seed=11)
+    param = make_param(["Fe", "Cu"], ["Fe", "Cu"], rng.uniform(100, 300, K), rng.uniform(100, 300, (K, K)),
+                       rho=1.0, pix=1e-4)
+    angles = phantom.angle_list(A)
+    cone = Cone(n=3, half_angle_deg=15.0, q=8)
+    model = XRFModel(param, angles, cone, store=False)
+    Ct = torch.from_numpy(C_true).cuda()
+    I = model.forward(Ct)
+    C0 = torch.full_like(Ct, float(Ct.mean()))
+    lr = 0.02 * float(C0.mean())
+    print(f"joint judgement: K={K} B={B} N={N} A={A}, 10 + 100 Adam steps vs 100 MLEM", flush=True)
+
+    C_start = mlem_start(model, I, C0, n_iter=10, per_line=True)
+    C_adam, hist = joint_reconstruct(model, I, C_start, n_steps=100, lr=lr)
+    C_plain = mlem_start(model, I, C0, n_iter=100, per_line=True)
+    r_start, r_adam = hist[0][2], hist[-1][2]
+    r_plain = data_residual(model.forward(C_plain), I)
+
+    # the same pipeline in FP64
+    mask = cone.splat_mask(N)
+    em = model.em_T.cpu().numpy().T.astype(np.float64)            # (A, K)
+    coef_x = model.coef_x.cpu().numpy().astype(np.float64)        # (A, K)
+    coef_xrf = model.coef_xrf.cpu().numpy().astype(np.float64)    # (A, K, K)
+    theta = model.theta.cpu().numpy()
+    I64 = I.cpu().double()
+    C0_64 = C0.cpu().double()
+    with torch.no_grad():
+        att = torch.empty((K, A, B, N, N), dtype=torch.float64)
+        for a, ang in enumerate(angles):
+            Fr = tm.rotate(C0_64, ang)
+            Ax = tm.incident(Fr * torch.as_tensor(coef_x[a])[:, None, None, None])
+            mu = torch.einsum("jbhw,je->ebhw", Fr, torch.as_tensor(coef_xrf[a]))
+            att[:, a] = tm.xrf(mu, torch.as_tensor(mask, dtype=torch.float64)) * Ax
+    att = att.numpy()
+
+    def mlem64(n):
+        return np.stack([oracle.mlem_batch_f64(C0_64[e].numpy()[None], att[e], em[:, e:e + 1].copy(), theta,
+                                               I64[e].numpy(), n, 0.0, 0.01)[0] for e in range(K)])
+
+    start64 = mlem64(10)
+    plain64 = mlem64(100)
+    C64 = torch.tensor(start64, dtype=torch.float64, requires_grad=True)
+    opt = torch.optim.Adam([C64], lr=lr)
+    for _ in range(100):
+        opt.zero_grad()
+        P = tm.forward(C64, angles, coef_x, coef_xrf, em, mask).clamp_min(1e-6)
+        (P - I64 * torch.log(P)).mean().backward()
+        opt.step()
+        with torch.no_grad():
+            C64.clamp_(min=0.0)
+    e_start = rel_l2(C_start.cpu().numpy(), start64)
+    e_plain = rel_l2(C_plain.cpu().numpy(), plain64)
+    e_adam = rel_l2(C_adam.cpu().numpy(), C64.detach().numpy())
+    print(f"  data residual: start {r_start:.4f}, 100 Adam steps {r_adam:.4f}, 100 plain MLEM {r_plain:.4f}")
+    print(f"  vs FP64 oracle: start {e_start:.2e}, Adam {e_adam:.2e}, plain MLEM {e_plain:.2e}")
+    assert e_start <= 1e-5 and e_plain <= 1e-4
+    assert e_adam <= 1e-3
+    assert r_adam < r_start
