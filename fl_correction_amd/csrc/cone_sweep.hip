@@ -34,7 +34,7 @@ constexpr uint32_t kSweepOOB = 0x80000000u;  // a byte offset past every buffer 
 
 struct SweepSetup {
     int ndir, Q, reverse;
-    int cols, PS;          // LDS tile: row length and plane size (floats)
+    int cols, NP;          // LDS tile: row length, floats per (z, y) position (the planes, odd)
     int dzmin, dymin;      // lowest tap offsets in z and y
     int pz[kSweepMaxDir], py[kSweepMaxDir];  // slopes scaled to Q
     float w[kSweepMaxDir];
@@ -82,7 +82,7 @@ __global__ void cone_sweep_setup_kernel(SweepSetup s, SweepTable tb)
                 const int dz = sg * (z0 + a), dy = sg * (y0 + b);
                 // staged plane of x + sg*t relative to the voxel's plane
                 const int pl = s.reverse ? s.Q - t : t - 1;
-                out[n++] = make_int2(pl * s.PS + (dz - s.dzmin) * s.cols + (dy - s.dymin),
+                out[n++] = make_int2(((dz - s.dzmin) * s.cols + (dy - s.dymin)) * s.NP + pl,
                                      __float_as_int((float)(wz * wy)));
             }
     }
@@ -102,7 +102,7 @@ struct SweepArgs {
     int K, Z, X, Y, Q, ndir;
     int xs, nx, pos0;  // the block's first plane, its plane count, that plane's position
     int first, reverse;
-    int TZ, rows, cols, PS, planes, dzmin, dymin;
+    int TZ, rows, cols, PS, NP, planes, dzmin, dymin;
 };
 
 template <int Q, int R>
@@ -120,7 +120,7 @@ __global__ __launch_bounds__(512) void cone_sweep_kernel(SweepArgs p)
     const int xfirst = p.reverse ? p.xs - Q : p.xs + 1;
     // the direction table, after the tile in LDS: its reads then wait on the
     // LDS counter only, not behind the ring's loads and stores
-    int2 *tab = reinterpret_cast<int2 *>(win + ((p.planes * p.PS + 3) & ~3));  // 16-byte aligned
+    int2 *tab = reinterpret_cast<int2 *>(win + ((p.NP * p.PS + 3) & ~3));  // 16-byte aligned
     int4 *dirs = reinterpret_cast<int4 *>(tab + p.ndir * 4 * Q);  // (count, shift z, shift y, weight bits)
     for (int i = threadIdx.x; i < p.ndir * 4 * Q; i += blockDim.x) tab[i] = p.tb.taps[i];
     for (int d = threadIdx.x; d < p.ndir; d += blockDim.x) {
@@ -129,19 +129,22 @@ __global__ __launch_bounds__(512) void cone_sweep_kernel(SweepArgs p)
     }
     // buffer loads: an offset past the line's volume reads 0 (no branches)
     const __amdgpu_buffer_rsrc_t rmu = make_rsrc(muk, (int)ZXY);
+    // the planes of one (z, y) position are neighbours in LDS (NP floats, odd, so
+    // that 64 lanes at consecutive y hit 64 different banks): a tap's R planes are
+    // then read as pairs of words
     const int total = p.planes * p.PS;
     for (int idx = threadIdx.x; idx < total; idx += blockDim.x) {
         const int pl = idx / p.PS, rem = idx - pl * p.PS;
         const int r = rem / p.cols, c = rem - r * p.cols;
         const int x = xfirst + pl, z = zt + p.dzmin + r, y = yt + p.dymin + c;
         const bool in = (unsigned)x < (unsigned)p.X && (unsigned)z < (unsigned)p.Z && (unsigned)y < (unsigned)p.Y;
-        win[idx] = load_f32(rmu, in ? (uint32_t)(((int64_t)z * p.X + x) * p.Y + y) * 4u : kSweepOOB);
+        win[rem * p.NP + pl] = load_f32(rmu, in ? (uint32_t)(((int64_t)z * p.X + x) * p.Y + y) * 4u : kSweepOOB);
     }
     __syncthreads();
 
     const int z = zt + zl, y = yt + lane;
     const bool act = z < p.Z && y < p.Y;
-    const float *wbase = win + zl * p.cols + lane;
+    const float *wbase = win + (zl * p.cols + lane) * p.NP;
     const int64_t ring_dir = (int64_t)Q * p.Z * p.Y;
     float Tacc[R];
 #pragma unroll
@@ -186,7 +189,7 @@ __global__ __launch_bounds__(512) void cone_sweep_kernel(SweepArgs p)
                 const float w = __int_as_float(t4[u].y);
                 const float *src = wbase + t4[u].x;
 #pragma unroll
-                for (int r = 0; r < R; ++r) S[r] = fmaf(w, src[r * p.PS], S[r]);
+                for (int r = 0; r < R; ++r) S[r] = fmaf(w, src[r], S[r]);
             }
         }
 #pragma unroll
@@ -212,7 +215,7 @@ __global__ __launch_bounds__(512) void cone_sweep_kernel(SweepArgs p)
 // Host side
 // ---------------------------------------------------------------------------
 struct SweepPlan {
-    int Q = 0, R = 0, TZ = 0, rows = 0, cols = 0, PS = 0, planes = 0, dzmin = 0, dymin = 0;
+    int Q = 0, R = 0, TZ = 0, rows = 0, cols = 0, PS = 0, NP = 0, planes = 0, dzmin = 0, dymin = 0;
     std::vector<int> pz, py;
 };
 
@@ -221,7 +224,7 @@ static size_t sweep_align(size_t bytes) { return (bytes + 255) / 256 * 256; }
 // LDS of one workgroup: the tile's planes, then the direction table.
 static size_t sweep_lds_bytes(const SweepPlan &pl, int ndir)
 {
-    return (size_t)((pl.planes * pl.PS + 3) & ~3) * sizeof(float) + (size_t)ndir * 4 * pl.Q * sizeof(int2) +
+    return (size_t)((pl.NP * pl.PS + 3) & ~3) * sizeof(float) + (size_t)ndir * 4 * pl.Q * sizeof(int2) +
            (size_t)ndir * sizeof(int4);
 }
 
@@ -261,7 +264,7 @@ static SweepPlan sweep_plan(const int32_t *pz, const int32_t *py, const int32_t 
             const int rows = TZ + (dzmax - dzmin);
             const int planes = R + Q - 1;
             const size_t table = (size_t)ndir * (4 * Q * sizeof(int2) + sizeof(int4));
-            if ((size_t)(planes * rows * pl.cols + 3) * sizeof(float) + table > 96 * 1024) continue;
+            if ((size_t)((planes | 1) * rows * pl.cols + 3) * sizeof(float) + table > 96 * 1024) continue;
             const double score = (double)TZ * R / ((double)planes * rows);
             if (score > best) {
                 best = score;
@@ -274,6 +277,7 @@ static SweepPlan sweep_plan(const int32_t *pz, const int32_t *py, const int32_t 
     if (!pl.TZ) return SweepPlan{};
     pl.Q = Q;
     pl.PS = pl.rows * pl.cols;
+    pl.NP = pl.planes | 1;
     pl.dzmin = dzmin;
     pl.dymin = dymin;
     return pl;
@@ -355,7 +359,7 @@ int launch_cone_sweep(const float *mu, float *T, int64_t K, int64_t Z, int64_t X
     su.Q = pl.Q;
     su.reverse = reverse;
     su.cols = pl.cols;
-    su.PS = pl.PS;
+    su.NP = pl.NP;
     su.dzmin = pl.dzmin;
     su.dymin = pl.dymin;
     for (int64_t d = 0; d < ndir; ++d) {
@@ -413,6 +417,7 @@ int launch_cone_sweep(const float *mu, float *T, int64_t K, int64_t Z, int64_t X
         a.rows = pl.rows;
         a.cols = pl.cols;
         a.PS = pl.PS;
+        a.NP = pl.NP;
         a.planes = pl.planes;
         a.dzmin = pl.dzmin;
         a.dymin = pl.dymin;
