@@ -88,7 +88,7 @@ __global__ __launch_bounds__(256) void mask_corr_kernel(const float *__restrict_
 
 // ---------------------------------------------------------------------------
 // Rotation: F.affine_grid([[c,-s,0],[s,c,0]]) + F.grid_sample(bilinear, zeros,
-// align_corners=False), as torch_rotate_image_nd (torch_image_util.py:337-438).
+// align_corners=False), as torch_rotate_image_nd (FL/torch_image_util.py:13-114).
 // ---------------------------------------------------------------------------
 __global__ __launch_bounds__(256) void rotate_kernel(const float *__restrict__ x, float *__restrict__ out, int64_t N,
                                                      int H, int W, float c, float s)

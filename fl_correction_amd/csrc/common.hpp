@@ -42,7 +42,7 @@ __device__ __forceinline__ float clampf(float v, float lo, float hi)
 __device__ __forceinline__ float finite_or_zero(float v) { return isfinite(v) ? v : 0.0f; }
 
 // Sample point of output pixel (i, j) for torch's affine_grid([[c,-s,0],[s,c,0]])
-// + grid_sample(align_corners=False) (FL/torch_image_util.py:401-428): the
+// + grid_sample(align_corners=False) (FL/torch_image_util.py:77-104): the
 // input position (iy, ix) split into its integer corner (y0, x0) and fractions.
 // Shared by the rotation and its transpose so both use identical arithmetic.
 __device__ __forceinline__ void rot_sample(int i, int j, int H, int W, float c, float s, int &y0, int &x0,

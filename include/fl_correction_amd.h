@@ -262,7 +262,7 @@ FL_API int fl_cal_atten(const float *mu, const float *mask, float *out, int64_t 
 
 /* In-plane rotation of N planes (H,W): F.affine_grid([[c,-s,0],[s,c,0]]) then
  * F.grid_sample (bilinear, zero padding, align_corners=False).  Replaces
- * torch_rotate_image_nd with its defaults (FL/torch_image_util.py:337-438).
+ * torch_rotate_image_nd with its defaults (FL/torch_image_util.py:13-114).
  * cos_t, sin_t: the float32 matrix entries (the reference rounds math.cos(theta)). */
 FL_API int fl_rotate(const float *x, float *out, int64_t N, int64_t H, int64_t W, float cos_t,
                      float sin_t, void *stream);
