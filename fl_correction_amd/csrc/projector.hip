@@ -1018,7 +1018,7 @@ __global__ __launch_bounds__(256) void bwd_tile_kernel(BwdArgs p)
 // tiles of one workgroup (the next tile's LDS writes wait for the previous
 // tile's cross-wave reads).
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(256, 2) void fwd_group_kernel(const FwdArgs *jobs, const uint32_t *begin, int n_jobs,
+__global__ __launch_bounds__(256, 4) void fwd_group_kernel(const FwdArgs *jobs, const uint32_t *begin, int n_jobs,
                                                            uint32_t n_tiles)
 {
     const uint32_t w = xcd_remap(blockIdx.x, gridDim.x);
