@@ -24,7 +24,9 @@ Besides the timed steps, the JSON line carries
     with a 5x5 detector cone over 15 deg, timed over all 360 angles, and the
     cone march's roofline (flop count and its own HBM traffic);
   * cpu_baseline: the oracle's CPU restatement of the same iteration, timed on
-    one line on the host cores and scaled to all lines.
+    one line on the host cores and scaled to all lines; beside it, at the
+    reference's CPU-sized case (64^3, 2 lines, 60 angles), the legacy dense-matrix
+    path and the oracle's pure-torch model run by torch on the host cores.
 
 --gpus above 1 is refused: this build runs on one GPU.
 """
@@ -301,6 +303,8 @@ def main():
     if not args.no_cpu_baseline:
         result["cpu_baseline"] = cpu_baseline(args)
         result["cpu_baseline"]["legacy_dense"] = legacy_dense_small(args)
+        result["cpu_baseline"]["torch_cpu"] = torch_cpu_small(
+            result["cpu_baseline"]["legacy_dense"]["gpu_ms_per_iteration"])
         if "model" in result:
             # the iteration that recomputes the attenuation for every angle first
             m = result["model"]
@@ -754,6 +758,101 @@ def legacy_dense_small(args, N=64, A=60, K=2, n_slices=8):
             "speedup_per_iteration": cpu_it / (gpu_ms * 1e-3), "cores": ncores,
             "sample": f"{n_slices} of {N} slices of one line, median of 3 runs, scaled x{scale:g}; FP64 dense "
                       f"matrix of {A * N} x {N * N} per slice; per iteration = (3 iterations - 1 iteration) / 2"}
+
+
+def torch_cpu_small(gpu_mlem_ms, n_att=10):
+    """The pure-torch restatement of the model in the oracle (oracle/torch_model.py:
+    the reference's own torch ops for the rotation, the incident product and the
+    mask correlation; the projector as an explicit gather), run by torch on all
+    host cores in FP32 at the reference's CPU-sized case: 64^3, 2 lines, 60 angles,
+    3x3 cone over 10 deg.  Timed: the attenuation maps (n_att angles, scaled to
+    all), and one MLEM iteration of both lines (projection, its transpose through
+    autograd, the update; the sensitivity untimed, as on the GPU), each the median
+    of 3 runs.  Beside them on the GPU: the model's attenuation maps of every angle
+    at the same size, and the MLEM iteration of the legacy leg (same sizes)."""
+    import torch
+    from fl_correction_amd.workloads import PLAIN_64 as wl
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import torch_model as tm  # the oracle's pure-torch restatement, timed as the torch-CPU path
+
+    ncores = len(os.sched_getaffinity(0))
+    threads0 = torch.get_num_threads()
+    torch.set_num_threads(ncores)
+    K, (B, H, W) = wl.K, wl.shape
+    angles = wl.angles()
+    A = len(angles)
+    C = torch.from_numpy(wl.phantom())
+    cx, cxrf = (torch.from_numpy(c) for c in wl.coefs())
+    mask = torch.from_numpy(np.asarray(wl.cone().splat_mask(W), np.float32))
+
+    def atten_angle(ang):
+        Fr = tm.rotate(C, ang)
+        Ax = tm.incident(Fr * cx[:, None, None, None])
+        mu = torch.einsum("jbhw,je->ebhw", Fr, cxrf)
+        return tm.xrf(mu, mask) * Ax
+
+    runs = []
+    for _ in range(3):
+        t0 = time.perf_counter()
+        for ang in angles[:n_att]:
+            atten_angle(ang)
+        runs.append(time.perf_counter() - t0)
+    cpu_att = float(np.median(runs)) * A / n_att
+
+    # MLEM on attenuation of the same shape (the cost does not depend on its values)
+    gen = torch.Generator().manual_seed(20260000)
+    att = torch.rand((K, A, B, H, W), generator=gen) * 0.7 + 0.3
+
+    def project_all(k, Cv):
+        return torch.stack([tm.project(att[k, a], Cv, 1.0, ang) for a, ang in enumerate(angles)])
+
+    I = [project_all(k, C[k]) for k in range(K)]
+    sens = []
+    for k in range(K):
+        Cv = torch.zeros((B, H, W), requires_grad=True)
+        P = project_all(k, Cv)
+        sens.append(torch.autograd.grad(P, Cv, torch.ones_like(P))[0].clamp_min(1e-6))
+
+    def mlem_iteration(Cs):
+        out = []
+        for k in range(K):
+            Cv = Cs[k].clone().requires_grad_(True)
+            P = project_all(k, Cv)
+            back = torch.autograd.grad(P, Cv, I[k] / P.detach().clamp_min(1e-6))[0]
+            out.append(Cs[k] * back / sens[k])
+        return torch.stack(out)
+
+    Cs = torch.ones((K, B, H, W))
+    runs = []
+    for _ in range(3):
+        t0 = time.perf_counter()
+        Cs = mlem_iteration(Cs)
+        runs.append(time.perf_counter() - t0)
+    cpu_it = float(np.median(runs))
+    torch.set_num_threads(threads0)
+
+    dev = torch.device("cuda:0")
+    model = wl.model()
+    Cd = C.to(dev)
+    model.attenuation(Cd)
+    torch.cuda.synchronize()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
+    model.attenuation(Cd)
+    e1.record()
+    torch.cuda.synchronize()
+    gpu_att_ms = e0.elapsed_time(e1)
+    log(f"torch CPU at {B}^3 x {A} x {K}: attenuation {cpu_att:.2f}s (GPU {gpu_att_ms:.2f} ms), "
+        f"MLEM iteration {cpu_it:.3f}s (GPU {gpu_mlem_ms:.3f} ms), {ncores} threads")
+    return {"workload": f"{B}^3 voxels x {A} angles x {K} lines, {wl.cone_n}x{wl.cone_n} cone over "
+                        f"{wl.cone_half:g} deg (the reference's CPU-sized case)",
+            "cpu_attenuation_s": cpu_att, "gpu_attenuation_ms": gpu_att_ms,
+            "speedup_attenuation": cpu_att / (gpu_att_ms * 1e-3),
+            "cpu_s_per_iteration": cpu_it, "gpu_ms_per_iteration": gpu_mlem_ms,
+            "speedup_per_iteration": cpu_it / (gpu_mlem_ms * 1e-3), "cores": ncores, "dtype": "f32",
+            "sample": f"torch {torch.__version__} on the host, FP32, {ncores} threads, median of 3 runs; attenuation "
+                      f"{n_att} of {A} angles scaled x{A / n_att:g} (mask correlation with the cone's splat mask); "
+                      f"MLEM one iteration of both lines, the transpose through autograd"}
 
 
 if __name__ == "__main__":
