@@ -180,7 +180,7 @@ def test_attenuation_of_selected_lines_and_per_line_mlem_start():
 
 
 def test_joint_reconstruction_against_fp64_oracle_and_plain_mlem():
-    """The joint-reconstruction judgement at a small size: a 10-iteration MLEM
+    """Joint reconstruction against plain MLEM at a small size: a 10-iteration MLEM
     start, then 100 Adam steps with C >= 0, beside 100 plain MLEM iterations on
     the start's attenuation.  Both GPU results against the same pipeline in FP64
     on the CPU (the torch FP64 model with the cone's splat mask, autograd and
@@ -205,7 +205,7 @@ def test_joint_reconstruction_against_fp64_oracle_and_plain_mlem():
     I = model.forward(Ct)
     C0 = torch.full_like(Ct, float(Ct.mean()))
     lr = 0.02 * float(C0.mean())
-    print(f"joint judgement: K={K} B={B} N={N} A={A}, 10 + 100 Adam steps vs 100 MLEM", flush=True)
+    print(f"joint vs MLEM: K={K} B={B} N={N} A={A}, 10 + 100 Adam steps vs 100 MLEM", flush=True)
 
     C_start = mlem_start(model, I, C0, n_iter=10, per_line=True)
     C_adam, hist = joint_reconstruct(model, I, C_start, n_steps=100, lr=lr)
