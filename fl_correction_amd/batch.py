@@ -59,13 +59,43 @@ def _check(p: Problem, i: int):
             raise ValueError(f"problem {i}: {name} must be a contiguous float32 tensor on {dev}")
 
 
-def _structs(problems, shapes):
+def _structs(problems, shapes, thetas=None):
     arr = (_lib.MLEMProblem * len(problems))()
     for i, (p, (fa, br)) in enumerate(zip(problems, shapes)):
         A, B, H, W = p.dims()
-        arr[i] = _lib.MLEMProblem(p.atten.data_ptr(), p.em_cs.data_ptr(), p.theta.data_ptr(), p.I.data_ptr(),
+        th = p.theta if thetas is None else thetas[i]
+        arr[i] = _lib.MLEMProblem(p.atten.data_ptr(), p.em_cs.data_ptr(), th.data_ptr(), p.I.data_ptr(),
                                   p.C.data_ptr(), A, B, H, W, int(fa), int(br))
     return arr
+
+
+PACK_MERGE_SLICES = 256  # slices up to which samples with equal angles share a pack
+
+
+def merged_thetas(problems, cap=PACK_MERGE_SLICES):
+    """The theta tensor each problem is planned with.  The plan packs problems
+    that share a theta tensor and (A, H, W) along the slice axis.  Here the
+    samples (the problems sharing one theta tensor) whose angles are equal value
+    for value, with equal (A, H, W), are given one tensor, in order, as long as
+    their slices total at most `cap`: more lanes of each wavefront then hold
+    slices.  Reads every distinct theta tensor once (host copies)."""
+    units = {}  # (theta pointer, H, W) -> [slices, (A, H, W), problem indices, theta]
+    for i, p in enumerate(problems):
+        A, B, H, W = p.dims()
+        u = units.setdefault((p.theta.data_ptr(), H, W), [0, (A, H, W), [], p.theta])
+        u[0] += B
+        u[2].append(i)
+    out = [p.theta for p in problems]
+    groups = {}  # (A, H, W, angle bytes) -> [canonical tensor, slices so far]
+    for slices, dims, members, theta in units.values():
+        key = dims + (theta.detach().cpu().numpy().tobytes(),)
+        g = groups.get(key)
+        if g is None or g[1] + slices > cap:
+            groups[key] = g = [theta, 0]
+        g[1] += slices
+        for i in members:
+            out[i] = g[0]
+    return out
 
 
 def workspace_bytes(problems, shapes=None):
@@ -83,8 +113,9 @@ class BatchMLEM:
     update (atten_cuda.cpp:330-341), as in ``mlem_cuda_batch``.
 
     layout: 'slice_major' packs the problems that share a theta tensor and
-    (A, H, W) -- the emission lines of one sample -- along the slice axis and
-    back-projects each pack from a slice-major copy of its attenuation (the
+    (A, H, W) -- the emission lines of one sample, and samples whose angles are
+    equal, up to PACK_MERGE_SLICES slices (merged_thetas) -- along the slice
+    axis and back-projects each pack from a slice-major copy of its attenuation (the
     workspace then holds that copy: about the attenuation's size again);
     'reference' back-projects every problem from its own attenuation; 'auto'
     takes 'slice_major' when its workspace fits the free device memory with
@@ -102,9 +133,10 @@ class BatchMLEM:
         if layout not in ("auto", "slice_major", "reference"):
             raise ValueError(f"layout must be 'auto', 'slice_major' or 'reference', not {layout!r}")
         self.problems = list(problems)  # keeps every array alive while the plan uses it
+        self._thetas = merged_thetas(self.problems)
         # the workspace sizes do not depend on the tile shapes
         lib = _lib.load()
-        arr = ctypes.cast(_structs(self.problems, [(0, 0)] * len(problems)), ctypes.c_void_p)
+        arr = ctypes.cast(_structs(self.problems, [(0, 0)] * len(problems), self._thetas), ctypes.c_void_p)
         nbytes = int(lib.fl_mlem_multi_workspace_bytes(arr, len(problems)))
         if nbytes == 0:
             raise ValueError("invalid batch")
@@ -122,7 +154,7 @@ class BatchMLEM:
         self.shapes = list(shapes) if shapes is not None else [(0, 0)] * len(problems)
         if len(self.shapes) != len(problems):
             raise ValueError("one tile shape per problem")
-        self._arr = _structs(self.problems, self.shapes)
+        self._arr = _structs(self.problems, self.shapes, self._thetas)
         # 256-byte aligned workspace inside a slightly larger buffer
         self._ws_buf = torch.empty(nbytes + 256, dtype=torch.uint8, device=self.device)
         off = (-self._ws_buf.data_ptr()) % 256

@@ -113,3 +113,32 @@ def test_autotuner_buckets_and_cache(tmp_path):
     assert not Autotuner.valid([4, 16]) and not Autotuner.valid([5, 31]) and not Autotuner.valid(None)
     (tmp_path / "bad.json").write_text("{not json")
     assert Autotuner(path=str(tmp_path / "bad.json")).table == {}
+
+
+def test_samples_with_equal_angles_share_a_pack():
+    """merged_thetas: samples (problems sharing a theta tensor) with equal angles
+    and (A, H, W) get one theta tensor, in order, up to PACK_MERGE_SLICES slices;
+    different angles, image sizes or an exceeded cap keep their own tensor."""
+    import torch
+    from fl_correction_amd.batch import PACK_MERGE_SLICES, Problem, merged_thetas
+
+    def sample(theta, lines, B, N):
+        A = theta.numel()
+        return [Problem(torch.zeros((A, B, N, N)), torch.ones(A), theta, torch.zeros((A, B, N)),
+                        torch.ones((B, N, N))) for _ in range(lines)]
+
+    ang = torch.linspace(0.0, 3.0, 30)
+    t1, t2, t3, t5 = ang.clone(), ang.clone(), ang.clone(), ang.clone()
+    t4 = ang.clone() + 0.01                 # other angles
+    probs = (sample(t1, 2, 64, 16) + sample(t2, 2, 64, 16)  # 256 slices: one pack
+             + sample(t3, 1, 40, 16)                        # would exceed the cap: its own
+             + sample(t4, 2, 8, 16)                         # other angles: its own
+             + sample(t5, 1, 8, 24))                        # other image size: its own
+    assert PACK_MERGE_SLICES == 256
+    got = [t.data_ptr() for t in merged_thetas(probs)]
+    want = [t1.data_ptr()] * 4 + [t3.data_ptr()] + [t4.data_ptr()] * 2 + [t5.data_ptr()]
+    assert got == want
+    # a sample of one line with the same angles as t3 joins t3's pack
+    t6 = ang.clone()
+    got = [t.data_ptr() for t in merged_thetas(probs + sample(t6, 1, 8, 16))]
+    assert got[-1] == t3.data_ptr()

@@ -122,6 +122,42 @@ def test_packed_lines_match_oracle(beta, n_iter):
     ref.close()
 
 
+def test_samples_with_equal_angles_share_a_pack():
+    """Samples whose angles are equal value for value, in separate theta tensors,
+    are planned as one pack (merged_thetas): every line against the oracle and
+    against the reference layout, which plans each problem on its own."""
+    A, H, W = 12, 30, 44
+    theta = np.random.default_rng(700).uniform(0, np.pi, A).astype(np.float32)
+    data, theta_of, sample_of = [], [], []
+    for g, Bs in enumerate([[5, 3], [7], [4, 6]]):
+        first = len(data)
+        for k, B in enumerate(Bs):
+            a, e, _, _, C0 = _problem(A, B, H, W, seed=710 + 10 * g + k)
+            C_true = np.random.default_rng(720 + 10 * g + k).uniform(0.0, 1.0, (1, B, H, W)).astype(np.float32)
+            I = oracle.forward_batch_f32(a, C_true, e[:, None], theta)
+            data.append((a, e, theta.copy(), I, C0))
+            theta_of.append(first)
+            sample_of.append(g)
+    print(f"equal angles: samples of {[3, 1, 2]} lines, A={A} {H}x{W}", flush=True)
+    sm, sm_probs = _batch(data, layout="slice_major", theta_of=theta_of)
+    ref, ref_probs = _batch(data, layout="reference", theta_of=theta_of)
+    # three samples, three theta tensors, one pack
+    assert len({p.theta.data_ptr() for p in sm_probs}) == 3
+    assert len({t.data_ptr() for t in sm._thetas}) == 1
+    sm.run(4, graph=True)
+    ref.run(4, graph=True)
+    torch.cuda.synchronize()
+    for i, (d, p, q) in enumerate(zip(data, sm_probs, ref_probs)):
+        a, e, t, I, C0 = d
+        want = oracle.mlem_batch_f32(C0[None], a, e[:, None], t, I, 4, 0.0, 0.01)[0]
+        got = p.C.cpu().numpy()
+        e32, er = rel_l2(got, want), rel_l2(got, q.C.cpu().numpy())
+        print(f"  line {i} (sample {sample_of[i]}) {a.shape}: vs f32 oracle {e32:.1e}, vs reference layout {er:.1e}")
+        assert e32 <= RTOL and er <= RTOL
+    sm.close()
+    ref.close()
+
+
 def test_auto_layout_takes_slice_major(data):
     run, _ = _batch(data[:2])
     assert run.layout == "slice_major"
