@@ -128,12 +128,21 @@ def joint_reconstruct(model: XRFModel, I, C0, n_steps=100, lr=None, eps=1e-6, ca
     2% of the start's mean value.  Returns (C, history) with history a list of
     (step, loss, data residual) before each update and after the last one.
     callback(step, C, loss) is called after every step if given.  `streams` is
-    the model's stream count for every forward and vjp."""
+    the model's stream count for every forward and vjp.
+
+    The loss reported is the reference's Poisson loss, a mean over the
+    measurements (FL/ml_lib/ml_loss.py:6-8).  Adam is given the gradient of
+    their sum (the mean's times I.numel()): Adam does not depend on the
+    gradient's scale except through its eps (1e-8), and the mean's gradient
+    shrinks with the number of measurements -- about 7e-11 per voxel at
+    128 x 512^2 x 720 angles x 5 lines, where eps would all but stop the
+    steps."""
     I = I.to(model.device, torch.float32).contiguous()
     C = C0.to(model.device, torch.float32).clone().contiguous().requires_grad_(True)
     if lr is None:
         lr = 0.02 * float(C0.mean().clamp_min(1e-6))
     opt = torch.optim.Adam([C], lr=lr)
+    n_meas = float(I.numel())
     history = []
     for step in range(int(n_steps) + 1):
         # loss, gradient and projections in one pass over the angles (the cone
@@ -142,7 +151,7 @@ def joint_reconstruct(model: XRFModel, I, C0, n_steps=100, lr=None, eps=1e-6, ca
         history.append((step, float(loss), data_residual(Pf, I)))
         if step == n_steps:
             break
-        C.grad = grad
+        C.grad = grad * n_meas
         opt.step()
         with torch.no_grad():
             C.clamp_(min=0.0)

@@ -241,7 +241,7 @@ def test_joint_reconstruction_against_fp64_oracle_and_plain_mlem():
     for _ in range(100):
         opt.zero_grad()
         P = tm.forward(C64, angles, coef_x, coef_xrf, em, mask).clamp_min(1e-6)
-        (P - I64 * torch.log(P)).mean().backward()
+        (P - I64 * torch.log(P)).sum().backward()  # Adam on the summed loss, as joint_reconstruct
         opt.step()
         with torch.no_grad():
             C64.clamp_(min=0.0)
