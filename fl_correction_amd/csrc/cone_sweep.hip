@@ -15,7 +15,10 @@
 // planes (R = Q unless the stencil's halo makes the staging too large): it
 // stages the mu planes its taps reach (its planes shifted by 1..Q, with the
 // stencil's halo in z and y) in LDS once, then computes every direction's
-// stencil from LDS.  Each direction's J of the block goes to a
+// stencil from LDS: per direction, each (z, y) column its taps touch is read
+// once as a window of R + Q - 1 planes into registers, and the column's weights
+// (one per staged plane, read through the scalar cache) are applied to all R
+// planes from there.  Each direction's J of the block goes to a
 // ring in device memory, read by the next block's launch at the shifted
 // position; T accumulates in registers over the directions in order and is
 // written once.  The chain form (cone_kernel in atten.hip) instead reads and
@@ -40,11 +43,13 @@ struct SweepSetup {
     float w[kSweepMaxDir];
 };
 
-// Per direction: up to 4Q taps (LDS offset, weight) at d * 4Q, their count (a
-// multiple of 4), the shift of the recursion (signed for the mode) and the
-// direction's weight.
+// Per direction: the columns of the tile its taps reach, at most 4Q, each as
+// Q + 1 words at d * 4Q * (Q + 1): the LDS offset of the column's staged plane
+// 0, then the bilinear weight of the tap on each staged plane (0 where the
+// direction has none); their count, the shift of the recursion (signed for the
+// mode) and the direction's weight.
 struct SweepTable {
-    int2 *taps;
+    int *cols;
     int *count;
     int2 *shift;
     float *weight;
@@ -63,14 +68,15 @@ __device__ __forceinline__ int floordiv_dev(int a, int b)
 }
 
 // One thread per direction: the bilinear taps of its Q samples, weights rounded
-// from their exact rational values, zero weights dropped.
+// from their exact rational values, zero weights dropped, gathered by column.
 __global__ void cone_sweep_setup_kernel(SweepSetup s, SweepTable tb)
 {
     const int d = threadIdx.x;
     if (d >= s.ndir) return;
     const int sg = s.reverse ? -1 : 1;
     int n = 0;
-    int2 *out = tb.taps + d * 4 * s.Q;
+    // columns of this direction: (LDS offset of staged plane 0, weight by staged plane)
+    int *out = tb.cols + d * 4 * s.Q * (s.Q + 1);
     for (int t = 1; t <= s.Q; ++t) {
         const int nz = t * s.pz[d], ny = t * s.py[d];
         const int z0 = floordiv_dev(nz, s.Q), y0 = floordiv_dev(ny, s.Q);
@@ -82,12 +88,17 @@ __global__ void cone_sweep_setup_kernel(SweepSetup s, SweepTable tb)
                 const int dz = sg * (z0 + a), dy = sg * (y0 + b);
                 // staged plane of x + sg*t relative to the voxel's plane
                 const int pl = s.reverse ? s.Q - t : t - 1;
-                out[n++] = make_int2(((dz - s.dzmin) * s.cols + (dy - s.dymin)) * s.NP + pl,
-                                     __float_as_int((float)(wz * wy)));
+                const int off = ((dz - s.dzmin) * s.cols + (dy - s.dymin)) * s.NP;
+                int c = 0;
+                while (c < n && out[c * (s.Q + 1)] != off) ++c;
+                if (c == n) {
+                    out[c * (s.Q + 1)] = off;
+                    for (int j = 0; j < s.Q; ++j) out[c * (s.Q + 1) + 1 + j] = 0;
+                    ++n;
+                }
+                out[c * (s.Q + 1) + 1 + pl] = __float_as_int((float)(wz * wy));
             }
     }
-    // zero-weight taps (reading the tile's first word) up to a multiple of 4
-    while (n % 4) out[n++] = make_int2(0, 0);
     tb.count[d] = n;
     tb.shift[d] = make_int2(sg * s.pz[d], sg * s.py[d]);
     tb.weight[d] = s.w[d];
@@ -118,11 +129,9 @@ __global__ __launch_bounds__(512) void cone_sweep_kernel(SweepArgs p)
     // ---- stage the planes the launch's taps reach ---------------------------
     // forward: x = xs+1 .. xs+R-1+Q; adjoint: x = xs-Q .. xs+R-2 (R + Q - 1 planes)
     const int xfirst = p.reverse ? p.xs - Q : p.xs + 1;
-    // the direction table, after the tile in LDS: its reads then wait on the
-    // LDS counter only, not behind the ring's loads and stores
-    int2 *tab = reinterpret_cast<int2 *>(win + ((p.NP * p.PS + 3) & ~3));  // 16-byte aligned
-    int4 *dirs = reinterpret_cast<int4 *>(tab + p.ndir * 4 * Q);  // (count, shift z, shift y, weight bits)
-    for (int i = threadIdx.x; i < p.ndir * 4 * Q; i += blockDim.x) tab[i] = p.tb.taps[i];
+    // the per-direction records, after the tile in LDS: their reads then wait
+    // on the LDS counter only, not behind the ring's loads and stores
+    int4 *dirs = reinterpret_cast<int4 *>(win + ((p.NP * p.PS + 3) & ~3));  // (count, shift z, shift y, weight bits)
     for (int d = threadIdx.x; d < p.ndir; d += blockDim.x) {
         const int2 sh = p.tb.shift[d];
         dirs[d] = make_int4(p.tb.count[d], sh.x, sh.y, __float_as_int(p.tb.weight[d]));
@@ -172,7 +181,8 @@ __global__ __launch_bounds__(512) void cone_sweep_kernel(SweepArgs p)
         const int4 di = dirs[d];
         const float wd = __int_as_float(di.w);
         const int cnt = di.x;
-        const int2 *taps = tab + d * 4 * Q;
+        using ctab_t = const __attribute__((address_space(4))) int *;
+        const ctab_t ct = (ctab_t)(p.tb.cols) + d * 4 * Q * (Q + 1);
         float *rout = p.ring_out + ((int64_t)k * p.ndir + d) * ring_dir;
         if (d + 1 < p.ndir) load_jp(d + 1, jn);
         // every staged plane r < R is computed (planes past nx are in the tile
@@ -180,16 +190,17 @@ __global__ __launch_bounds__(512) void cone_sweep_kernel(SweepArgs p)
         float S[R];
 #pragma unroll
         for (int r = 0; r < R; ++r) S[r] = 0.0f;
-        for (int i = 0; i < cnt; i += 4) {
-            int2 t4[4];
+        for (int c = 0; c < cnt; ++c) {
+            const ctab_t e = ct + c * (Q + 1);
+            const float *src = wbase + e[0];
+            float col[R + Q - 1];
 #pragma unroll
-            for (int u = 0; u < 4; ++u) t4[u] = taps[i + u];
+            for (int k = 0; k < R + Q - 1; ++k) col[k] = src[k];
 #pragma unroll
-            for (int u = 0; u < 4; ++u) {
-                const float w = __int_as_float(t4[u].y);
-                const float *src = wbase + t4[u].x;
+            for (int j = 0; j < Q; ++j) {
+                const float w = __int_as_float(e[1 + j]);
 #pragma unroll
-                for (int r = 0; r < R; ++r) S[r] = fmaf(w, src[r], S[r]);
+                for (int r = 0; r < R; ++r) S[r] = fmaf(w, col[j + r], S[r]);
             }
         }
 #pragma unroll
@@ -221,11 +232,10 @@ struct SweepPlan {
 
 static size_t sweep_align(size_t bytes) { return (bytes + 255) / 256 * 256; }
 
-// LDS of one workgroup: the tile's planes, then the direction table.
+// LDS of one workgroup: the tile's planes, then the per-direction records.
 static size_t sweep_lds_bytes(const SweepPlan &pl, int ndir)
 {
-    return (size_t)((pl.NP * pl.PS + 3) & ~3) * sizeof(float) + (size_t)ndir * 4 * pl.Q * sizeof(int2) +
-           (size_t)ndir * sizeof(int4);
+    return (size_t)((pl.NP * pl.PS + 3) & ~3) * sizeof(float) + (size_t)ndir * sizeof(int4);
 }
 
 // Geometry of the sweep for a direction set; Q = 0 when it does not apply.
@@ -263,7 +273,7 @@ static SweepPlan sweep_plan(const int32_t *pz, const int32_t *py, const int32_t 
         for (int TZ : {8, 4, 2, 1}) {
             const int rows = TZ + (dzmax - dzmin);
             const int planes = R + Q - 1;
-            const size_t table = (size_t)ndir * (4 * Q * sizeof(int2) + sizeof(int4));
+            const size_t table = (size_t)ndir * sizeof(int4);
             if ((size_t)((planes | 1) * rows * pl.cols + 3) * sizeof(float) + table > 96 * 1024) continue;
             const double score = (double)TZ * R / ((double)planes * rows);
             if (score > best) {
@@ -290,7 +300,7 @@ size_t cone_sweep_workspace_bytes(int64_t K, int64_t Z, int64_t X, int64_t Y, co
     const SweepPlan pl = sweep_plan(pz, py, q, ndir, 0);
     if (!pl.Q) return 0;
     const size_t ring = sweep_align((size_t)K * ndir * pl.Q * Z * Y * sizeof(float));
-    const size_t table = sweep_align((size_t)ndir * 4 * pl.Q * sizeof(int2)) + sweep_align(ndir * sizeof(int)) +
+    const size_t table = sweep_align((size_t)ndir * 4 * pl.Q * (pl.Q + 1) * sizeof(int)) + sweep_align(ndir * sizeof(int)) +
                          sweep_align(ndir * sizeof(int2)) + sweep_align(ndir * sizeof(float));
     return 2 * ring + table;
 }
@@ -346,8 +356,8 @@ int launch_cone_sweep(const float *mu, float *T, int64_t K, int64_t Z, int64_t X
     float *rings[2] = {(float *)ws, (float *)(ws + ring)};
     char *tp = ws + 2 * ring;
     SweepTable tb;
-    tb.taps = (int2 *)tp;
-    tp += sweep_align((size_t)ndir * 4 * pl.Q * sizeof(int2));
+    tb.cols = (int *)tp;
+    tp += sweep_align((size_t)ndir * 4 * pl.Q * (pl.Q + 1) * sizeof(int));
     tb.count = (int *)tp;
     tp += sweep_align(ndir * sizeof(int));
     tb.shift = (int2 *)tp;
