@@ -71,6 +71,9 @@ __device__ __forceinline__ int floordiv_dev(int a, int b)
 // from their exact rational values, zero weights dropped, gathered by column.
 __global__ void cone_sweep_setup_kernel(SweepSetup s, SweepTable tb)
 {
+    // offsets of the columns found so far (column-major: neighbouring lanes in
+    // different banks), searched here rather than in the table just written
+    __shared__ int found[4 * 16][kSweepMaxDir];
     const int d = threadIdx.x;
     if (d >= s.ndir) return;
     const int sg = s.reverse ? -1 : 1;
@@ -90,8 +93,9 @@ __global__ void cone_sweep_setup_kernel(SweepSetup s, SweepTable tb)
                 const int pl = s.reverse ? s.Q - t : t - 1;
                 const int off = ((dz - s.dzmin) * s.cols + (dy - s.dymin)) * s.NP;
                 int c = 0;
-                while (c < n && out[c * (s.Q + 1)] != off) ++c;
+                while (c < n && found[c][d] != off) ++c;
                 if (c == n) {
+                    found[c][d] = off;
                     out[c * (s.Q + 1)] = off;
                     for (int j = 0; j < s.Q; ++j) out[c * (s.Q + 1) + 1 + j] = 0;
                     ++n;
