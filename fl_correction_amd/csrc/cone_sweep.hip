@@ -31,6 +31,8 @@
 
 namespace fl {
 
+#include "cone_fixed.inc"  // tile geometry of the compiled-in direction grids
+
 constexpr int kSweepMaxDir = 64;
 constexpr int kSweepLanes = 64;
 constexpr uint32_t kSweepOOB = 0x80000000u;  // a byte offset past every buffer (reads 0)
@@ -120,8 +122,11 @@ struct SweepArgs {
     int TZ, rows, cols, PS, NP, planes, dzmin, dymin;
 };
 
-template <int Q, int R>
-__global__ __launch_bounds__(512) void cone_sweep_kernel(SweepArgs p)
+// FM > 0: the direction grid {-FM..FM}^2 / 8 of cone_fixed.inc, whose stencil is
+// compiled in (Q = R = 8, the tile geometry the plan makes for it).
+template <int Q, int R, int FM = 0>
+__global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(FM ? 4 : 1))) void cone_sweep_kernel(
+    SweepArgs p)
 {
     extern __shared__ float win[];
     const int k = blockIdx.z;
@@ -179,34 +184,10 @@ __global__ __launch_bounds__(512) void cone_sweep_kernel(SweepArgs p)
             jp[r] = load_f32(rin, ok ? (uint32_t)(((p.pos0 + r) * p.Z + zz) * p.Y + yy) * 4u : kSweepOOB);
         }
     };
-    float jp[R], jn[R];
-    load_jp(0, jp);
-    for (int d = 0; d < p.ndir; ++d) {
-        const int4 di = dirs[d];
-        const float wd = __int_as_float(di.w);
-        const int cnt = di.x;
-        using ctab_t = const __attribute__((address_space(4))) int *;
-        const ctab_t ct = (ctab_t)(p.tb.cols) + d * 4 * Q * (Q + 1);
+    // the end of direction d: J = S + J_prev to the ring, and T
+    auto finish = [&](int d, const float *S, const float *jp) {
+        const float wd = __int_as_float(dirs[d].w);
         float *rout = p.ring_out + ((int64_t)k * p.ndir + d) * ring_dir;
-        if (d + 1 < p.ndir) load_jp(d + 1, jn);
-        // every staged plane r < R is computed (planes past nx are in the tile
-        // and are not written): no branch between the LDS reads
-        float S[R];
-#pragma unroll
-        for (int r = 0; r < R; ++r) S[r] = 0.0f;
-        for (int c = 0; c < cnt; ++c) {
-            const ctab_t e = ct + c * (Q + 1);
-            const float *src = wbase + e[0];
-            float col[R + Q - 1];
-#pragma unroll
-            for (int k = 0; k < R + Q - 1; ++k) col[k] = src[k];
-#pragma unroll
-            for (int j = 0; j < Q; ++j) {
-                const float w = __int_as_float(e[1 + j]);
-#pragma unroll
-                for (int r = 0; r < R; ++r) S[r] = fmaf(w, col[j + r], S[r]);
-            }
-        }
 #pragma unroll
         for (int r = 0; r < R; ++r) {
             if (r < p.nx) {
@@ -215,8 +196,92 @@ __global__ __launch_bounds__(512) void cone_sweep_kernel(SweepArgs p)
                 Tacc[r] = d == 0 ? wd * J : fmaf(wd, J, Tacc[r]);
             }
         }
-#pragma unroll
-        for (int r = 0; r < R; ++r) jp[r] = jn[r];
+    };
+    if constexpr (FM == 0) {
+        float jp[R], jn[R];
+        load_jp(0, jp);
+        for (int d = 0; d < p.ndir; ++d) {
+            const int4 di = dirs[d];
+            const float wd = __int_as_float(di.w);
+            const int cnt = di.x;
+            using ctab_t = const __attribute__((address_space(4))) int *;
+            const ctab_t ct = (ctab_t)(p.tb.cols) + d * 4 * Q * (Q + 1);
+            float *rout = p.ring_out + ((int64_t)k * p.ndir + d) * ring_dir;
+            if (d + 1 < p.ndir) load_jp(d + 1, jn);
+            // every staged plane r < R is computed (planes past nx are in the tile
+            // and are not written): no branch between the LDS reads
+            float S[R];
+    #pragma unroll
+            for (int r = 0; r < R; ++r) S[r] = 0.0f;
+            for (int c = 0; c < cnt; ++c) {
+                const ctab_t e = ct + c * (Q + 1);
+                const float *src = wbase + e[0];
+                float col[R + Q - 1];
+    #pragma unroll
+                for (int k = 0; k < R + Q - 1; ++k) col[k] = src[k];
+    #pragma unroll
+                for (int j = 0; j < Q; ++j) {
+                    const float w = __int_as_float(e[1 + j]);
+    #pragma unroll
+                    for (int r = 0; r < R; ++r) S[r] = fmaf(w, col[j + r], S[r]);
+                }
+            }
+    #pragma unroll
+            for (int r = 0; r < R; ++r) {
+                if (r < p.nx) {
+                    const float J = S[r] + jp[r];
+                    if (act) rout[((int64_t)(p.pos0 + r) * p.Z + z) * p.Y + y] = J;
+                    Tacc[r] = d == 0 ? wd * J : fmaf(wd, J, Tacc[r]);
+                }
+            }
+    #pragma unroll
+            for (int r = 0; r < R; ++r) jp[r] = jn[r];
+        }
+    } else {
+        // Groups of up to G directions with one pz: each column the group's taps
+        // touch is read once (a window of R + Q - 1 planes) and its taps, with
+        // literal weights, add into their directions' plane sums.
+        constexpr int G = 3;  // at most G directions per group (gen_cone_fixed.py GMAX)
+#define CF_GROUP_BEGIN(d0)                                      \
+    {                                                           \
+        float S[G][R], c[R + Q - 1];                            \
+        _Pragma("unroll") for (int g = 0; g < G; ++g)           \
+            _Pragma("unroll") for (int r = 0; r < R; ++r) S[g][r] = 0.0f;
+#define CF_COL(off) _Pragma("unroll") for (int i = 0; i < R + Q - 1; ++i) c[i] = wbase[(off) + i];
+#define CF_TAP(g, j, w) _Pragma("unroll") for (int r = 0; r < R; ++r) S[g][r] = fmaf((w), c[(j) + r], S[g][r]);
+#define CF_GROUP_END(d0, n)                                     \
+        _Pragma("unroll") for (int g = 0; g < (n); ++g)         \
+        {                                                       \
+            float jg[R];                                        \
+            load_jp((d0) + g, jg);                              \
+            finish((d0) + g, S[g], jg);                         \
+        }                                                       \
+    }
+        if constexpr (FM == 1) {
+            if (!p.reverse) {
+#define CF_M1_FWD_BODY
+#include "cone_fixed.inc"
+#undef CF_M1_FWD_BODY
+            } else {
+#define CF_M1_ADJ_BODY
+#include "cone_fixed.inc"
+#undef CF_M1_ADJ_BODY
+            }
+        } else {
+            if (!p.reverse) {
+#define CF_M2_FWD_BODY
+#include "cone_fixed.inc"
+#undef CF_M2_FWD_BODY
+            } else {
+#define CF_M2_ADJ_BODY
+#include "cone_fixed.inc"
+#undef CF_M2_ADJ_BODY
+            }
+        }
+#undef CF_GROUP_BEGIN
+#undef CF_COL
+#undef CF_TAP
+#undef CF_GROUP_END
     }
     if (act) {
         float *Tk = p.T + k * ZXY;
@@ -315,10 +380,30 @@ static const void *sweep_fn()
     return (const void *)cone_sweep_kernel<Q, R>;
 }
 
-template <int Q, int R>
+template <int Q, int R, int FM = 0>
 static void sweep_launch(const SweepArgs &a, dim3 grid, int threads, size_t lds, hipStream_t s)
 {
-    cone_sweep_kernel<Q, R><<<grid, dim3(threads), lds, s>>>(a);
+    cone_sweep_kernel<Q, R, FM><<<grid, dim3(threads), lds, s>>>(a);
+}
+
+// 1 or 2: the plan's directions are the grid {-M..M}^2 / 8 of cone_fixed.inc,
+// in its order, with the tile geometry compiled in; else 0.
+static int fixed_grid(const SweepPlan &pl, int64_t ndir, int reverse)
+{
+    if (pl.Q != 8 || pl.R != 8 || pl.TZ != 8 || (ndir != 9 && ndir != 25)) return 0;
+    const int M = ndir == 9 ? 1 : 2, side = 2 * M + 1;
+    for (int64_t d = 0; d < ndir; ++d)
+        if (pl.pz[d] != (int)(d / side) - M || pl.py[d] != (int)(d % side) - M) return 0;
+    struct Geo {
+        int dzmin, dymin, cols, rows, np;
+    };
+    const Geo g = M == 1 ? (reverse ? Geo{CF_M1_ADJ_DZMIN, CF_M1_ADJ_DYMIN, CF_M1_ADJ_COLS, CF_M1_ADJ_ROWS, CF_M1_ADJ_NP}
+                                    : Geo{CF_M1_FWD_DZMIN, CF_M1_FWD_DYMIN, CF_M1_FWD_COLS, CF_M1_FWD_ROWS, CF_M1_FWD_NP})
+                         : (reverse ? Geo{CF_M2_ADJ_DZMIN, CF_M2_ADJ_DYMIN, CF_M2_ADJ_COLS, CF_M2_ADJ_ROWS, CF_M2_ADJ_NP}
+                                    : Geo{CF_M2_FWD_DZMIN, CF_M2_FWD_DYMIN, CF_M2_FWD_COLS, CF_M2_FWD_ROWS, CF_M2_FWD_NP});
+    const bool same = pl.dzmin == g.dzmin && pl.dymin == g.dymin && pl.cols == g.cols && pl.rows == g.rows &&
+                      pl.NP == g.np;
+    return same ? M : 0;
 }
 
 // The kernel instance for (Q, R), R in {Q, Q/2, Q/4}.
@@ -387,12 +472,20 @@ int launch_cone_sweep(const float *mu, float *T, int64_t K, int64_t Z, int64_t X
     const size_t lds = sweep_lds_bytes(pl, (int)ndir);
     const void *fn = nullptr;
     void (*launch)(const SweepArgs &, dim3, int, size_t, hipStream_t) = nullptr;
-    if (pl.Q == 4)
+    const int fm = fixed_grid(pl, ndir, reverse);
+    if (fm == 1) {
+        fn = (const void *)cone_sweep_kernel<8, 8, 1>;
+        launch = sweep_launch<8, 8, 1>;
+    } else if (fm == 2) {
+        fn = (const void *)cone_sweep_kernel<8, 8, 2>;
+        launch = sweep_launch<8, 8, 2>;
+    } else if (pl.Q == 4) {
         sweep_pick<4>(pl.R, &fn, &launch);
-    else if (pl.Q == 8)
+    } else if (pl.Q == 8) {
         sweep_pick<8>(pl.R, &fn, &launch);
-    else
+    } else {
         sweep_pick<16>(pl.R, &fn, &launch);
+    }
     FL_HIP_CHECK(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
     const dim3 grid((unsigned)((Y + kSweepLanes - 1) / kSweepLanes), (unsigned)((Z + pl.TZ - 1) / pl.TZ), (unsigned)K);
     const int nb = (int)((X + pl.Q - 1) / pl.Q);
