@@ -482,7 +482,8 @@ def cone_roofline(K, B, H, W, cone, reps=5):
     traffic = 4.0 * vox * (2 + 2 * cone.ndir)
     del mu, T, ws
     return {
-        "kernel": "cone_sweep_kernel<8,8> (X/Q launches: one angle, all lines)", "ms_per_angle": ms,
+        "kernel": "cone_sweep_kernel<8,8,2> (the 5x5 grid's compiled-in stencil; X/Q launches: one angle, all lines)",
+        "ms_per_angle": ms,
         "taps_per_voxel": taps,
         "flops": flops, "achieved_tflops": flops / ms / 1e9, "fp32_peak_tflops": FP32_PEAK_TFLOPS,
         "flop_frac": flops / ms / 1e9 / FP32_PEAK_TFLOPS,
